@@ -1,0 +1,157 @@
+#!/usr/bin/env python3
+"""Benchmark: Metropolis moves/s and pair-evals/s on the 12,288-atom water box.
+
+Workload (one GPU): 4,096 rigid TIP3P-like waters (12,288 atoms) in a cubic
+box of 49.7 A, switched Coulomb+LJ (10/12 A), 2 A list skin, T = 300 K,
+residue TRN (0.3 A) and ROT (15 deg) moves.  A "step" is a window of
+`--moves` Metropolis moves run by the device loop (per-move dE, accept test
+and commit on the GPU), with the Verlet list rebuilt on the GPU every 50
+moves as the reference schedules it.  value = moves per second.
+
+Also measured in the same run (HIP events on the library's own stream):
+  * the O(N^2) pair kernel (full min-image LJ+Coulomb energy of the box):
+    pair-evals/s and FP32 fraction of peak -> "roofline";
+  * the list build: algorithmic HBM bytes/s -> "list_build";
+  * the reference's own CPU code (oracle/_ref/libmcref.so, compiled from the
+    mc_cpp sources) on this host: moves/s of its two-getE-per-move loop
+    on a bounded sample -> "cpu_baseline".
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+FLOP_FULL_PAIR = 32.0          # per pair, FMA = 2 (the constant frozen in csrc/mcx_internal.h)
+
+
+def cpu_reference_baseline(s, fp, nmoves, seed):
+    """Times the reference's MC_metropolis::run (two full getE per move) on
+    a bounded sample; returns (moves_per_s, seconds, threads)."""
+    from oracle import oracle
+    from mc_cpp_amd import systems
+    if not oracle.ref_available():
+        return None
+    d = tempfile.mkdtemp(prefix="mcx_bench_")
+    cor, ff = systems.save_mdbas(os.path.join(d, "water"), s, fp)
+    r = oracle.Reference(cor, ff, s)
+    t0 = time.perf_counter()
+    r.mc_run(s.moves, nmoves, seed)
+    dt = time.perf_counter() - t0
+    return nmoves / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--moves", type=int, default=20000, help="Metropolis moves per step")
+    ap.add_argument("--cpu-moves", type=int, default=24, help="moves of the reference CPU loop to time")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the Metropolis chain is sequential and this build has no "
+              "multi-GPU path; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from mc_cpp_amd import mcx, systems
+
+    s, fp = systems.water_box(seed=2)
+    s_cpu = s.copy()
+    ff = mcx.FField(s)
+    h = ff.h
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:
+        sync = lambda: None
+
+    # ---- headline: device Metropolis loop ----
+    for w in range(args.warmup):
+        h.mc_run(s.moves, args.moves, 1000 + w, rng_mode=mcx.MC_REPLAY, list_every=50, record=False)
+    sync()
+    t0 = time.perf_counter()
+    acc = 0
+    nlist = 0
+    for k in range(args.steps):
+        r = h.mc_run(s.moves, args.moves, 2000 + k, rng_mode=mcx.MC_REPLAY, list_every=50, record=False)
+        acc += int(r["accepted"].sum())
+        nlist += int(r["list_updates"])
+    sync()
+    dt = time.perf_counter() - t0
+    moves_per_s = args.steps * args.moves / dt
+
+    # ---- pair kernel: full O(N^2) energy, HIP events on the library stream ----
+    n = s.natom
+    s_full = s.copy()
+    s_full.cutmode = systems.CUT_FULL
+    hf = mcx.from_system(s_full)
+    hf.energy()
+    reps = 50
+    ms_full = hf.time_kernel(0, reps)
+    pairs = n * (n - 1) / 2
+    flop = FLOP_FULL_PAIR * pairs
+    tflops = flop / (ms_full * 1e-3) / 1e12
+
+    # ---- switched list energy kernel and list build ----
+    ms_sw = h.time_kernel(1, reps)
+    ms_list = h.time_kernel(2, 20)
+    npairs = h.update_list()
+    list_bytes = 24 * n + 4 * (n + 1) + 4 * npairs
+    list_gbs = list_bytes / (ms_list * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        res = cpu_reference_baseline(s_cpu, fp, args.cpu_moves, 7)
+        if res is not None:
+            mps, secs = res
+            cpu = {"value": mps, "unit": "moves/s", "cores": 1, "kind": "reference",
+                   "sample": "%d moves of MC_metropolis::run on this box (two full switched getE per move, "
+                             "single-threaded energy loops as the reference builds them), %.1f s"
+                             % (args.cpu_moves, secs)}
+
+    out = {
+        "metric": "MC moves/s, pair-evals/s (%FP32 peak) on 12k-atom water; batched vs plain",
+        "value": moves_per_s,
+        "unit": "moves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair math, f64 Coulomb distance chain and sums",
+        "data": "synthetic (seeded Philox water lattice, random orientations)",
+        "config": {"workload": "water_box_12288: 4096 rigid TIP3P-like waters, L=49.7 A, switch 10/12 A, "
+                               "skin 2 A, T=300 K, residue TRN 0.3 A + ROT 15 deg, plain single-chain "
+                               "device Metropolis loop, list rebuilt every 50 moves",
+                   "moves_per_step": args.moves, "acceptance": acc / (args.steps * args.moves),
+                   "list_rebuilds": nlist},
+        "pair_evals_per_s": pairs / (ms_full * 1e-3),
+        "full_energy_ms": ms_full,
+        "switch_energy_ms": ms_sw,
+        "roofline": {"bound": "valu", "kernel": "k_full_energy (O(N^2) min-image LJ+Coulomb, 12,288 atoms)",
+                     "achieved": tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": tflops / PEAK_FP32_TFLOPS, "traffic": None,
+                     "flop_per_launch": flop, "ms_per_launch": ms_full},
+        "list_build": {"bound": "hbm", "achieved": list_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                       "frac": list_gbs / PEAK_HBM_GBS, "bytes_per_launch": list_bytes, "ms_per_launch": ms_list,
+                       "pairs": npairs},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,182 @@
+/*
+ * mcx.h -- C ABI of the MI355X (gfx950) energy backend for mc_cpp.
+ *
+ * One opaque handle per simulated system (or per batch of replicas).  All
+ * arrays are plain host pointers that the library copies; the caller keeps
+ * ownership.  Every call returns an int status (MCX_OK on success); the
+ * message of the last failure on a handle is read with mcx_last_error().
+ * Calls on one handle are serialised on that handle's HIP stream; distinct
+ * handles are independent.
+ *
+ * Units are the reference's internal units (src/IO_MDBAS.cpp:130-505,
+ * include/Constants.hpp:29): energies in 10 J/mol (1 kcal/mol = 418.4 iu),
+ * eps = sqrt(eps_kcal*418.4), sig = sigma/2, bonded k already *418.4,
+ * angles in radians.
+ *
+ * Reference interfaces replaced (file:line in the mc_cpp tree):
+ *   FField ctor / cut mode        include/FField.hpp:58-59, src/FField.cpp:27-51
+ *   PerConditions                 src/PerConditions.cpp:103-134
+ *   IO_MDBAS -> AtomList / FField::set*List   src/IO_MDBAS.cpp:130-505, src/FField.cpp:127-180
+ *   List_nonBonded                include/List_nonBonded.hpp:50-59, src/List_nonBonded.cpp:143-915
+ *   FField::getE / getE(double[10]) include/FField.hpp:95-96, src/FField_MDBAS.cpp:53-106
+ *   FField::askListUpdate         include/FField.hpp:98, src/FField.cpp:222-235
+ *   MC_metropolis::run            src/MC_metropolis.cpp:45-204
+ *   FORTRAN C ABI (kcal/mol)      src/FORTRAN_interface.cpp:21-106
+ */
+#ifndef MCX_H
+#define MCX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCX_OK                 0
+#define MCX_ERR_ARG            1  /* bad argument or size */
+#define MCX_ERR_HIP            2  /* HIP runtime failure */
+#define MCX_ERR_LIST_OVERFLOW  3  /* pair-list capacity exceeded (the library regrows and rebuilds) */
+#define MCX_ERR_BOX_TOO_SMALL  4  /* cutoff + skin exceeds half the box in fast (non-compat) paths */
+#define MCX_ERR_STATE          5  /* call order violated (e.g. energy before coordinates) */
+#define MCX_ERR_UNSUPPORTED    6  /* configuration outside what the device path implements */
+
+#define MCX_PBC_NONE   0
+#define MCX_PBC_CUBIC  1
+#define MCX_PBC_ORBIC  2
+
+#define MCX_CUT_FULL   0
+#define MCX_CUT_SWITCH 1
+
+#define MCX_MV_TRN 1
+#define MCX_MV_ROT 2
+#define MCX_MODE_RESIDUE 1
+#define MCX_MODE_ALL     2
+#define MCX_MODE_ATOM    4
+
+typedef struct mcx_ctx mcx_ctx;
+
+/* Creates a handle for natom atoms on HIP device `device`.
+ * Replaces the FField/AtomList/Ensemble construction of Parser_XML
+ * (src/Parser.cpp:77-215). */
+int mcx_create(int natom, int device, mcx_ctx **out);
+void mcx_destroy(mcx_ctx *h);
+const char *mcx_last_error(const mcx_ctx *h);
+
+/* PerConditions(type, a, b, c) -- src/PerConditions.cpp:31-61,103-134.
+ * For MCX_PBC_CUBIC only `a` is used. */
+int mcx_set_box(mcx_ctx *h, int pbtype, double a, double b, double c);
+
+/* FField(..., cutMode, ctoff, cuton, dcut) -- include/FField.hpp:58-59;
+ * temperature of Ens_NVT (src/Ens_NVT.cpp:23-31) used by the Metropolis test. */
+int mcx_set_cut(mcx_ctx *h, int cutmode, double ctoff, double cton, double dcut, double temp);
+
+/* AtomList per-atom arrays as IO_MDBAS::read_ff fills them
+ * (src/IO_MDBAS.cpp:183-186,460-465; src/IO_MDBAS.cpp:121 for resid).
+ * frozen: 0/1.  resid: residue_id_global (1-based, consecutive). */
+int mcx_set_atoms(mcx_ctx *h, const double *charge, const double *eps, const double *sig,
+                  const double *eps14, const double *sig14, const double *mass,
+                  const int *frozen, const int *resid);
+
+/* FField::setBndList/setUbList/setAngList/setDiheList/setImprList
+ * (src/FField.cpp:157-180) with the records of include/{Bond,Bond_UB,Angle,
+ * Dihedral,Dihedral_improper}.hpp.  Atom indices 0-based. */
+int mcx_set_bonds(mcx_ctx *h, int n, const int *a, const int *b, const int *type,
+                  const double *k, const double *r0, const double *beta);
+int mcx_set_ub(mcx_ctx *h, int n, const int *a, const int *b, const double *k, const double *r0);
+int mcx_set_angles(mcx_ctx *h, int n, const int *a, const int *b, const int *c,
+                   const double *k, const double *theta0);
+int mcx_set_dihedrals(mcx_ctx *h, int n, const int *a, const int *b, const int *c, const int *d,
+                      const int *type, const double *k, const double *phi0, const double *mult);
+int mcx_set_impropers(mcx_ctx *h, int n, const int *a, const int *b, const int *c, const int *d,
+                      const int *type, const double *k, const double *phi0, const double *mult);
+
+/* List_nonBonded::build_exclude_list (src/List_nonBonded.cpp:143-775):
+ * builds the exclusion rows and the 1-4 pair list on the host from the
+ * bonded lists and uploads them.  Must follow the mcx_set_{bonds..} calls. */
+int mcx_build_exclusions(mcx_ctx *h);
+
+/* getExclPair/getExclList/getNPair14/getNeighList14
+ * (src/List_nonBonded.cpp:1343-1361).  Rows are written back to back into
+ * `idx` (visible entries only); returns sizes through *nidx, *npair14. */
+int mcx_get_exclusions(mcx_ctx *h, int *exclPair, int *idx, long long cap, long long *nidx,
+                       int *pairs14, int cap14, int *npair14);
+
+/* AtomList::setCoords / getCoords for all atoms (src/FORTRAN_interface.cpp:77-82). */
+int mcx_set_coords(mcx_ctx *h, const double *x, const double *y, const double *z);
+int mcx_get_coords(mcx_ctx *h, double *x, double *y, double *z);
+
+/* List_nonBonded::update_verlet_list (src/List_nonBonded.cpp:836-915) on the
+ * GPU: cell binning + in-kernel PerConditions minimum image + stream
+ * compaction.  The pair set equals the reference's bit for bit.  Reports
+ * the number of (i<j) pairs.  FField::askListUpdate (src/FField.cpp:222-235)
+ * maps to this call; it is a no-op in MCX_CUT_FULL mode. */
+int mcx_update_list(mcx_ctx *h, long long *npairs);
+
+/* getNeighPair/getNeighList (src/List_nonBonded.cpp:1363-1376): the half
+ * list, row i holding ascending j > i, written back to back into `flat`. */
+int mcx_get_list(mcx_ctx *h, int *neighPair, int *flat, long long cap);
+
+/* FField::getE(double ener[10]) (src/FField_MDBAS.cpp:77-106):
+ * {tot, pot, kin, elec, vdw, bond, ang, ub, dihe, impr} in iu. */
+int mcx_energy(mcx_ctx *h, double ener[10]);
+
+/* Energy change of moving atoms [first,last] to the given coordinates
+ * (arrays of last-first+1 entries), against the current list; equal to
+ * getE(new) - getE(old) as MC_metropolis::run forms it
+ * (src/MC_metropolis.cpp:94,122-123).  Coordinates are not committed. */
+int mcx_delta_energy(mcx_ctx *h, int first, int last, const double *nx, const double *ny,
+                     const double *nz, double *de);
+
+/* One Monte Carlo move type, as List_Moves::addNewMoveType registers it
+ * (src/List_Moves.cpp:170-211,257-457).  sele: 0/1 per atom or NULL = all. */
+typedef struct mcx_move_type {
+    int type;          /* MCX_MV_TRN / MCX_MV_ROT */
+    int mode;          /* MCX_MODE_RESIDUE / MCX_MODE_ATOM / MCX_MODE_ALL */
+    double dmax;       /* A for TRN, degrees for ROT */
+    double target;     /* acceptance target in % */
+    int each;          /* dmax update period in steps, 0 = never */
+    const int *sele;
+} mcx_move_type;
+
+typedef struct mcx_mc_stats {
+    double etot;           /* running total after the last step (iu) */
+    long long trials[8];
+    long long accepted[8];
+    double dmax[8];        /* final dmax per move type */
+    long long list_updates;
+    double seconds;        /* wall time of the device loop */
+} mcx_mc_stats;
+
+#define MCX_MC_REPLAY   1  /* draw from std::mt19937(seed) + generate_canonical exactly as src/MC.cpp:119-156 */
+#define MCX_MC_PHILOX   0  /* counter-based Philox4x32-10 proposals (no replay) */
+
+/* MC_metropolis::run (src/MC_metropolis.cpp:45-173) with the per-move dE
+ * evaluated on the device and the Metropolis test of apply_criterion
+ * (:175-204), including its beta = 1/((0.83144621/418.4)*T) applied to dE
+ * in iu.  The list is rebuilt every `list_every` steps (the reference uses
+ * 50, :151-154).  Optional per-step outputs (nsteps entries each): accepted
+ * flag, dE, move type, move instance. */
+int mcx_mc_run(mcx_ctx *h, const mcx_move_type *types, int ntypes, int nsteps, uint64_t seed,
+               int rng_mode, int list_every, mcx_mc_stats *stats, unsigned char *acc_out,
+               double *de_out, int *mvtyp_out, int *mvinst_out);
+
+/* Adjoint of the switched nonbonded energy (list-based): dE/dx,dy,dz per
+ * atom and dE/deps, dE/dsig, dE/dq per atom (iu, with eps/sig in the
+ * AtomList convention).  Not in the reference. */
+int mcx_nonbonded_grad(mcx_ctx *h, double *gx, double *gy, double *gz,
+                       double *geps, double *gsig, double *gq);
+
+/* Device pointer access for zero-copy interop (PyTorch tensors, HIP code):
+ * which = 0 x, 1 y, 2 z (double[N] master coordinates). */
+int mcx_device_coords(mcx_ctx *h, int which, void **ptr);
+
+/* Timing probe used by the benchmark: runs `reps` back-to-back launches of
+ * kernel `which` (0 full O(N^2) energy, 1 switched list energy, 2 list
+ * build, 3 fused bonded) on the handle's stream and returns the mean
+ * device time per launch from HIP events. */
+int mcx_time_kernel(mcx_ctx *h, int which, int reps, double *ms_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1018 @@
+// mcx_api.cpp -- host side of the gfx950 energy backend: the C ABI of
+// include/mcx.h, device buffer management, the exclusion-list builder and
+// the move-instance tables (both host work in the reference too).
+#include "../../include/mcx.h"
+#include "mcx_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <memory>
+
+namespace {
+
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count)
+    {
+        if (count <= n && p) return;
+        if (p) HIPCHK(hipFree(p));
+        p = nullptr;
+        n = 0;
+        if (count == 0) count = 1;
+        HIPCHK(hipMalloc((void **)&p, count * sizeof(T)));
+        n = count;
+    }
+    void upload(const T *h, size_t count, hipStream_t st)
+    {
+        alloc(count);
+        if (count) HIPCHK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, st));
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+}  // namespace
+
+struct mcx_ctx {
+    int n = 0, device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // parameters
+    int pbtype = MCX_PBC_CUBIC;
+    double box[3] = {0, 0, 0};
+    int cutmode = MCX_CUT_SWITCH;
+    double ctoff = 12.0, cton = 10.0, dcut = 2.0, temp = 300.0;
+    bool have_atoms = false, have_coords = false, have_excl = false, list_valid = false;
+    std::vector<double> q, eps, sig, eps14, sig14, mass;
+    std::vector<int> frozen, resid;
+    HostTerms terms;    // kinds 0..4 from the caller, 5/6 from the exclusion builder
+    std::vector<int> excl_pair, excl_off, excl_idx;   // visible rows
+    std::vector<int> pairs14;
+    // device buffers
+    DBuf<double> x, y, z, q64, eps64, sig64, eps1464, sig1464, mass64;
+    DBuf<int4> ipos;
+    DBuf<float2> es, esk, es14;
+    std::vector<double> hx, hy, hz;   // host copy of the last uploaded coordinates
+    DBuf<int> frozen_d, excl_off_d, excl_idx_d;
+    DBuf<int> nbr, cnt, half, overflow;
+    int cap = 0;
+    DBuf<int> cell_of, cell_start, cell_count, cell_atoms, cursor;
+    CellGrid grid{};
+    DBuf<int2> tiles;
+    int ntiles = 0;
+    DBuf<double> partial, result;
+    DBuf<int4> t_idx[kNumKinds];
+    DBuf<int> t_type[kNumKinds];
+    DBuf<double4> t_par[kNumKinds];
+    DBuf<int> adj_off, adj_term;
+    DBuf<int> mv_first, mv_last, mv_pivot;
+    DBuf<double> newxyz, scalar;
+    long long last_npairs = 0;
+
+    DevSys sys() const
+    {
+        DevSys s{};
+        s.n = n;
+        s.pbtype = pbtype;
+        for (int k = 0; k < 3; k++) {
+            s.L[k] = box[k];
+            s.rL[k] = pbtype == MCX_PBC_NONE ? 0.0 : 1.0 / box[k];
+            s.Lf[k] = (float)s.L[k];
+            s.rLf[k] = (float)s.rL[k];
+        }
+        s.ctoff2 = ctoff * ctoff;
+        s.cton2 = cton * cton;
+        const double dd = s.ctoff2 - s.cton2;
+        s.switch2 = 1. / (dd * dd * dd);
+        double cn = ctoff + dcut;
+        s.cutnb2 = cn * cn;
+        s.cutmode = cutmode;
+        s.x = x.p; s.y = y.p; s.z = z.p;
+        s.ipos = ipos.p;
+        s.es = es.p; s.esk = esk.p; s.es14 = es14.p;
+        for (int k = 0; k < 3; k++) s.unit[k] = pbtype == MCX_PBC_NONE ? 1.0 : box[k] / 4294967296.0;
+        s.iunit = pbtype == MCX_PBC_CUBIC ? 4294967296.0 / box[0] : 1.0;
+        s.q64 = q64.p; s.eps64 = eps64.p; s.sig64 = sig64.p; s.eps1464 = eps1464.p; s.sig1464 = sig1464.p;
+        s.frozen = frozen_d.p;
+        s.excl_off = excl_off_d.p; s.excl_idx = excl_idx_d.p;
+        s.cap = cap;
+        s.nbr = nbr.p; s.cnt = cnt.p; s.half = half.p; s.overflow = overflow.p;
+        return s;
+    }
+
+    DevTerms dterms() const
+    {
+        DevTerms t{};
+        t.start[0] = 0;
+        for (int k = 0; k < kNumKinds; k++) {
+            t.nterm[k] = (int)terms.idx[k].size();
+            t.start[k + 1] = t.start[k] + t.nterm[k];
+            t.idx[k] = t_idx[k].p;
+            t.type[k] = t_type[k].p;
+            t.par[k] = t_par[k].p;
+        }
+        // the excluded-pair correction applies to FULL mode only
+        if (cutmode != MCX_CUT_FULL) {
+            t.nterm[kExclPair] = 0;
+            t.start[kExclPair + 1] = t.start[kExclPair];
+        }
+        return t;
+    }
+
+    ~mcx_ctx()
+    {
+        DBuf<double> *d[] = {&x, &y, &z, &q64, &eps64, &sig64, &eps1464, &sig1464, &mass64, &partial, &result,
+                             &newxyz, &scalar};
+        for (auto *b : d) b->release();
+        ipos.release(); es.release(); esk.release(); es14.release();
+        DBuf<int> *ib[] = {&frozen_d, &excl_off_d, &excl_idx_d, &nbr, &cnt, &half, &overflow, &cell_of,
+                           &cell_start, &cell_count, &cell_atoms, &cursor, &adj_off, &adj_term, &mv_first,
+                           &mv_last, &mv_pivot};
+        for (auto *b : ib) b->release();
+        tiles.release();
+        for (int k = 0; k < kNumKinds; k++) { t_idx[k].release(); t_type[k].release(); t_par[k].release(); }
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int fail(mcx_ctx *h, int code, const std::string &msg)
+{
+    if (h) h->err = msg;
+    return code;
+}
+
+template <typename F>
+int guarded(mcx_ctx *h, F &&f)
+{
+    if (!h) return MCX_ERR_ARG;
+    try {
+        HIPCHK(hipSetDevice(h->device));
+        f();
+        return MCX_OK;
+    } catch (const mcx_error &e) {
+        return fail(h, e.code, e.msg);
+    } catch (const std::exception &e) {
+        return fail(h, MCX_ERR_ARG, e.what());
+    }
+}
+
+// ---- exclusion lists: List_nonBonded::build_exclude_list restated
+// (src/List_nonBonded.cpp:143-775) ----
+void build_exclusions_host(mcx_ctx *h)
+{
+    const int N = h->n;
+    std::vector<std::vector<int>> tmp(N), con(N);
+    std::vector<int> v14;
+    auto add = [&](int a, int b) -> bool {
+        for (int e : tmp[a]) if (e == b) return false;
+        tmp[a].push_back(b);
+        tmp[b].push_back(a);
+        return true;
+    };
+    const HostTerms &T = h->terms;
+    for (const int4 &b : T.idx[kBond]) {     // excl_bonds :184-213
+        con[b.x].push_back(b.y); con[b.y].push_back(b.x);
+        tmp[b.x].push_back(b.y); tmp[b.y].push_back(b.x);
+    }
+    for (const int4 &a : T.idx[kAngle]) { add(a.x, a.y); add(a.x, a.z); add(a.y, a.z); }   // :215-299
+    for (const int4 &d : T.idx[kDihe]) {                                                   // :301-462
+        add(d.x, d.y); add(d.x, d.z);
+        if (add(d.x, d.w)) { v14.push_back(d.x); v14.push_back(d.w); }
+        add(d.y, d.z); add(d.y, d.w); add(d.z, d.w);
+    }
+    for (const int4 &d : T.idx[kImpr]) {                                                   // :464-619
+        add(d.x, d.y); add(d.x, d.z); add(d.x, d.w); add(d.y, d.z); add(d.y, d.w); add(d.z, d.w);
+    }
+    for (int ia = 0; ia < N; ia++)                                                          // :621-703
+        for (int ib : con[ia])
+            for (int ic : con[ib]) {
+                if (ic == ia) continue;
+                add(ia, ic);
+                for (int id : con[ic]) {
+                    if (id == ia || id == ib) continue;
+                    if (add(ia, id)) { v14.push_back(ia); v14.push_back(id); }
+                }
+            }
+    // excl_final_Lists :705-766 -- circular half rule, sorted insertion, and
+    // the rotation of entries < i, which drops them from the visible row
+    // and repeats the row's last entry
+    const int hn = N / 2, hm1 = (N - 1) / 2;
+    std::vector<std::vector<int>> rows(N);
+    h->excl_pair.assign(N, 0);
+    for (int i = 0; i < N; i++) {
+        std::vector<int> &L = rows[i];
+        L.assign(tmp[i].size() + 1, 0);
+        int jj = 0;
+        for (int e : tmp[i]) {
+            if (((e > i) && ((e - i) < hn)) || ((e < i) && ((e - i + N) < hm1))) {
+                L[jj] = e;
+                for (int k = jj; k > 0; k--)
+                    if (L[k] < L[k - 1]) std::swap(L[k], L[k - 1]);
+                jj++;
+            }
+        }
+        h->excl_pair[i] = jj;
+        for (int r = 0; r < jj; r++) {
+            if (L[0] < i) {
+                const int e = L[0];
+                for (int kk = 0; kk < jj - 1; kk++) L[kk] = L[kk + 1];
+                L[jj] = e;
+            }
+        }
+    }
+    h->excl_off.assign(N + 1, 0);
+    h->excl_idx.clear();
+    for (int i = 0; i < N; i++) {
+        h->excl_off[i + 1] = h->excl_off[i] + h->excl_pair[i];
+        h->excl_idx.insert(h->excl_idx.end(), rows[i].begin(), rows[i].begin() + h->excl_pair[i]);
+    }
+    h->pairs14 = v14;
+    // pair terms for the fused kernel: 1-4 pairs, and the distinct excluded
+    // pairs (i < j) whose all-pair contribution FULL mode must remove
+    HostTerms &W = h->terms;
+    W.idx[kPair14].clear(); W.type[kPair14].clear(); W.par[kPair14].clear();
+    for (size_t k = 0; k + 1 < v14.size(); k += 2) {
+        W.idx[kPair14].push_back(make_int4(v14[k], v14[k + 1], -1, -1));
+        W.type[kPair14].push_back(0);
+        W.par[kPair14].push_back(make_double4(0, 0, 0, 0));
+    }
+    W.idx[kExclPair].clear(); W.type[kExclPair].clear(); W.par[kExclPair].clear();
+    for (int i = 0; i < N; i++) {
+        int prev = -1;
+        for (int k = h->excl_off[i]; k < h->excl_off[i + 1]; k++) {
+            const int e = h->excl_idx[k];
+            if (e <= i || e == prev) continue;
+            prev = e;
+            W.idx[kExclPair].push_back(make_int4(i, e, -1, -1));
+            W.type[kExclPair].push_back(0);
+            W.par[kExclPair].push_back(make_double4(0, 0, 0, 0));
+        }
+    }
+}
+
+void upload_terms(mcx_ctx *h)
+{
+    for (int k = 0; k < kNumKinds; k++) {
+        h->t_idx[k].upload(h->terms.idx[k].data(), h->terms.idx[k].size(), h->stream);
+        h->t_type[k].upload(h->terms.type[k].data(), h->terms.type[k].size(), h->stream);
+        h->t_par[k].upload(h->terms.par[k].data(), h->terms.par[k].size(), h->stream);
+    }
+    // per-atom adjacency of bonded terms and 1-4 pairs (global work ids)
+    const int N = h->n;
+    std::vector<std::vector<int>> adj(N);
+    int base = 0;
+    for (int k = 0; k <= kPair14; k++) {
+        const auto &v = h->terms.idx[k];
+        for (size_t t = 0; t < v.size(); t++) {
+            const int ids[4] = {v[t].x, v[t].y, v[t].z, v[t].w};
+            for (int q = 0; q < 4; q++) {
+                const int a = ids[q];
+                if (a < 0) continue;
+                bool dup = false;
+                for (int p = 0; p < q; p++) dup |= ids[p] == a;
+                if (!dup) adj[a].push_back(base + (int)t);
+            }
+        }
+        base += (int)v.size();
+    }
+    std::vector<int> off(N + 1, 0), flat;
+    for (int i = 0; i < N; i++) {
+        off[i + 1] = off[i] + (int)adj[i].size();
+        flat.insert(flat.end(), adj[i].begin(), adj[i].end());
+    }
+    h->adj_off.upload(off.data(), off.size(), h->stream);
+    h->adj_term.upload(flat.data(), flat.size(), h->stream);
+    h->excl_off_d.upload(h->excl_off.data(), h->excl_off.size(), h->stream);
+    h->excl_idx_d.upload(h->excl_idx.data(), h->excl_idx.size(), h->stream);
+}
+
+void setup_grid(mcx_ctx *h)
+{
+    CellGrid &g = h->grid;
+    const double cutnb = h->ctoff + h->dcut;
+    const int reach = 2;
+    for (int d = 0; d < 3; d++) {
+        g.reach[d] = reach;
+        if (h->pbtype == MCX_PBC_NONE) { g.nc[d] = 1; continue; }
+        const double wmin = cutnb / reach * (1.0 + 1e-9) + 1e-6;
+        int nc = (int)std::floor(h->box[d] / wmin);
+        g.nc[d] = std::max(1, std::min(nc, 1024));
+    }
+    g.ncell = g.nc[0] * g.nc[1] * g.nc[2];
+    h->cell_of.alloc(h->n);
+    h->cell_atoms.alloc(h->n);
+    h->cell_start.alloc(g.ncell);
+    h->cell_count.alloc(g.ncell);
+    h->cursor.alloc(g.ncell);
+    g.cell_of = h->cell_of.p;
+    g.cell_atoms = h->cell_atoms.p;
+    g.cell_start = h->cell_start.p;
+    g.cell_count = h->cell_count.p;
+    g.cursor = h->cursor.p;
+}
+
+void ensure_list_capacity(mcx_ctx *h, int cap)
+{
+    cap = (cap + 31) & ~31;
+    if (cap <= h->cap && h->nbr.p) return;
+    h->cap = cap;
+    h->nbr.alloc((size_t)h->n * cap);
+    h->cnt.alloc(h->n);
+    h->half.alloc(h->n);
+    h->overflow.alloc(1);
+}
+
+long long build_list(mcx_ctx *h)
+{
+    if (!h->have_coords || !h->have_excl) throw mcx_error(MCX_ERR_STATE, "coordinates and exclusions must be set before the list");
+    if ((size_t)((h->n + 31) / 32) * 4 > 160 * 1024) throw mcx_error(MCX_ERR_UNSUPPORTED, "list build supports up to 1.3M atoms");
+    setup_grid(h);
+    if (h->cap == 0) {
+        double vol = h->box[0] * h->box[1] * h->box[2];
+        double rho = h->pbtype == MCX_PBC_NONE || vol <= 0 ? 0.1 : h->n / vol;
+        double rc = h->ctoff + h->dcut;
+        int est = (int)(rho * 4.18879 * rc * rc * rc * 1.3) + 64;
+        ensure_list_capacity(h, std::min(est, h->n));
+    }
+    for (int attempt = 0; attempt < 4; attempt++) {
+        DevSys s = h->sys();
+        launch_list_build(s, h->grid, h->stream);
+        HIPCHK(hipGetLastError());
+        int ov = 0;
+        HIPCHK(hipMemcpyAsync(&ov, h->overflow.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if (ov == 0) {
+            h->list_valid = true;
+            // count the half pairs
+            std::vector<int> c(h->n), hf(h->n);
+            HIPCHK(hipMemcpy(c.data(), h->cnt.p, sizeof(int) * h->n, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(hf.data(), h->half.p, sizeof(int) * h->n, hipMemcpyDeviceToHost));
+            long long np = 0;
+            for (int i = 0; i < h->n; i++) np += c[i] - hf[i];
+            h->last_npairs = np;
+            return np;
+        }
+        ensure_list_capacity(h, (int)(ov * 1.25) + 32);
+    }
+    throw mcx_error(MCX_ERR_LIST_OVERFLOW, "pair list capacity could not be grown");
+}
+
+// list rebuild without the host read-back of counts (used inside MC runs)
+void rebuild_list_fast(mcx_ctx *h)
+{
+    DevSys s = h->sys();
+    launch_list_build(s, h->grid, h->stream);
+    int ov = 0;
+    HIPCHK(hipMemcpyAsync(&ov, h->overflow.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (ov) {
+        ensure_list_capacity(h, (int)(ov * 1.25) + 32);
+        build_list(h);
+    }
+}
+
+void setup_tiles(mcx_ctx *h)
+{
+    const int nt = (h->n + 255) / 256;
+    std::vector<int2> t;
+    for (int I = 0; I < nt; I++)
+        for (int J = I; J < nt; J++) t.push_back(make_int2(I, J));
+    h->ntiles = (int)t.size();
+    h->tiles.upload(t.data(), t.size(), h->stream);
+}
+
+void energy_device(mcx_ctx *h, double out9[9])
+{
+    if (!h->have_coords || !h->have_atoms || !h->have_excl)
+        throw mcx_error(MCX_ERR_STATE, "atoms, coordinates and exclusions must be set before getE");
+    DevSys s = h->sys();
+    DevTerms t = h->dterms();
+    int nb_nb;
+    h->result.alloc(16);
+    if (h->cutmode == MCX_CUT_FULL) {
+        if (h->ntiles == 0) setup_tiles(h);
+        nb_nb = h->ntiles;
+        h->partial.alloc((size_t)std::max(nb_nb * 2, bonded_blocks(t) * 7));
+        launch_full_energy(s, h->tiles.p, h->ntiles, 256, h->partial.p, h->stream);
+    } else {
+        if (!h->list_valid) build_list(h);
+        s = h->sys();
+        nb_nb = switch_energy_blocks(h->n);
+        h->partial.alloc((size_t)std::max(nb_nb * 2, bonded_blocks(t) * 7));
+        launch_switch_energy(s, h->partial.p, nb_nb, h->stream);
+    }
+    launch_reduce(h->partial.p, nb_nb, 2, h->result.p, h->stream);
+    const int nbb = bonded_blocks(t);
+    launch_bonded(s, t, h->partial.p, nbb, h->stream);
+    launch_reduce(h->partial.p, nbb, 7, h->result.p + 2, h->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out9, h->result.p, 9 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+}
+
+void assemble(const double r[9], double ener[10])
+{
+    const double C = MCX_CHGCHARMM * MCX_KCALTOIU;
+    const double elec = C * (r[0] + r[2]);
+    const double vdw = 4.0 * (r[1] + r[3]);
+    const double bond = r[4], ang = r[5], ub = r[6], dihe = r[7], impr = r[8];
+    const double pot = elec + vdw + bond + ang + ub + dihe + impr;
+    ener[0] = pot + 0.0; ener[1] = pot; ener[2] = 0.0; ener[3] = elec; ener[4] = vdw;
+    ener[5] = bond; ener[6] = ang; ener[7] = ub; ener[8] = dihe; ener[9] = impr;
+}
+
+// fractional coordinate -> 32-bit fixed point, as the device computes it
+int to_fixed(double x, double rL)
+{
+    double v = x * rL;
+    v = v - std::floor(v);
+    return (int)(unsigned)(long long)(v * 4294967296.0);
+}
+
+void upload_mirror(mcx_ctx *h)
+{
+    const int n = h->n;
+    if (!h->have_coords) return;
+    std::vector<int4> m(n);
+    const bool pbc = h->pbtype != MCX_PBC_NONE;
+    for (int i = 0; i < n; i++) {
+        float q = h->have_atoms ? (float)h->q[i] : 0.f;
+        int qb;
+        std::memcpy(&qb, &q, 4);
+        if (pbc)
+            m[i] = make_int4(to_fixed(h->hx[i], 1.0 / h->box[0]), to_fixed(h->hy[i], 1.0 / h->box[1]),
+                             to_fixed(h->hz[i], 1.0 / h->box[2]), qb);
+        else
+            m[i] = make_int4(0, 0, 0, qb);
+    }
+    h->ipos.upload(m.data(), n, h->stream);
+    std::vector<float2> ek(n);
+    const double sc = h->pbtype == MCX_PBC_CUBIC ? 4294967296.0 / h->box[0] : 1.0;
+    for (int i = 0; i < n; i++) ek[i] = make_float2((float)h->eps[i], (float)(h->sig[i] * sc));
+    h->esk.upload(ek.data(), n, h->stream);
+    HIPCHK(hipStreamSynchronize(h->stream));
+}
+
+void upload_coords(mcx_ctx *h, const double *x, const double *y, const double *z)
+{
+    const int n = h->n;
+    h->x.upload(x, n, h->stream);
+    h->y.upload(y, n, h->stream);
+    h->z.upload(z, n, h->stream);
+    h->hx.assign(x, x + n); h->hy.assign(y, y + n); h->hz.assign(z, z + n);
+    h->have_coords = true;
+    h->list_valid = false;
+    upload_mirror(h);
+}
+
+// pull the device master coordinates back into the host copy (after MC runs)
+void sync_host_coords(mcx_ctx *h)
+{
+    const int n = h->n;
+    h->hx.resize(n); h->hy.resize(n); h->hz.resize(n);
+    HIPCHK(hipMemcpyAsync(h->hx.data(), h->x.p, n * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(h->hy.data(), h->y.p, n * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(h->hz.data(), h->z.p, n * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+}
+
+// List_Moves::NewMove_TRN_ROT instance tables (src/List_Moves.cpp:257-457,
+// gtrsfl :630-697): one instance per selected atom (RESIDUE: the residue
+// range, pivot = the atom for ROT; ATOM: the atom; ALL: one instance of the
+// whole contiguous selection with a centre-of-mass pivot).
+// gtrsfl's table: resLast[k] = last atom of residue k, resLast[0] = -1
+std::vector<int> residue_last_table(const mcx_ctx *h)
+{
+    const int N = h->n;
+    const int iresl = *std::max_element(h->resid.begin(), h->resid.end());
+    std::vector<int> resLast(iresl + 1);
+    resLast[0] = -1;
+    int last = 1, idx = 0;
+    for (int it = 0; it < N; it++) if (h->resid[it] > last) { last++; resLast[++idx] = it - 1; }
+    resLast[iresl] = N - 1;
+    return resLast;
+}
+
+// gtrsfl's binary search, bounds inclusive on both sides as in the
+// reference (src/List_Moves.cpp:659-685): for the last atom of a residue
+// the search may settle on the following residue, depending on its path.
+void residue_range(const std::vector<int> &resLast, int atomidx, int &a1, int &a2)
+{
+    int iresf = 1, iresl = (int)resLast.size() - 1;
+    while (iresf <= iresl) {
+        const int mid = (iresl + iresf) / 2;
+        if (resLast[mid - 1] <= atomidx && resLast[mid] >= atomidx) { a1 = resLast[mid - 1] + 1; a2 = resLast[mid]; return; }
+        else if (resLast[mid] <= atomidx) iresf = mid + 1;
+        else iresl = mid - 1;
+    }
+    throw mcx_error(MCX_ERR_ARG, "residue bounds not found");
+}
+
+DevMoves build_moves(mcx_ctx *h, const mcx_move_type *types, int ntypes)
+{
+    if (ntypes < 1 || ntypes > kMaxMoveTypes) throw mcx_error(MCX_ERR_ARG, "1..8 move types supported");
+    DevMoves mv{};
+    mv.ntypes = ntypes;
+    std::vector<int> F, L, P;
+    const std::vector<int> resLast = residue_last_table(h);
+    for (int t = 0; t < ntypes; t++) {
+        const mcx_move_type &m = types[t];
+        if (m.type != MCX_MV_TRN && m.type != MCX_MV_ROT) throw mcx_error(MCX_ERR_ARG, "move type must be TRN or ROT");
+        if (m.mode == MCX_MODE_ATOM && m.type == MCX_MV_ROT) throw mcx_error(MCX_ERR_ARG, "rotation of a single atom");
+        mv.type[t] = m.type;
+        mv.dmax0[t] = m.dmax;
+        mv.target[t] = m.target;
+        mv.each[t] = m.each;
+        mv.off[t] = (int)F.size();
+        if (m.mode == MCX_MODE_ALL) {
+            F.push_back(0); L.push_back(h->n - 1); P.push_back(-1);
+        } else {
+            for (int i = 0; i < h->n; i++) {
+                if (m.sele && !m.sele[i]) continue;
+                int a1 = i, a2 = i;
+                if (m.mode == MCX_MODE_RESIDUE) residue_range(resLast, i, a1, a2);
+                if (a2 - a1 + 1 > kMaxMoveAtoms)
+                    throw mcx_error(MCX_ERR_UNSUPPORTED, "move instance larger than 64 atoms");
+                F.push_back(a1); L.push_back(a2); P.push_back(m.type == MCX_MV_ROT ? i : -1);
+            }
+        }
+        mv.ninst[t] = (int)F.size() - mv.off[t];
+        if (mv.ninst[t] == 0) throw mcx_error(MCX_ERR_ARG, "empty move selection");
+        if (m.mode == MCX_MODE_ALL && h->n > kMaxMoveAtoms)
+            throw mcx_error(MCX_ERR_UNSUPPORTED, "ALL-mode moves of more than 64 atoms");
+    }
+    h->mv_first.upload(F.data(), F.size(), h->stream);
+    h->mv_last.upload(L.data(), L.size(), h->stream);
+    h->mv_pivot.upload(P.data(), P.size(), h->stream);
+    mv.first = h->mv_first.p; mv.last = h->mv_last.p; mv.pivot = h->mv_pivot.p;
+    mv.mass = h->mass64.p;
+    mv.adj_off = h->adj_off.p;
+    mv.adj_term = h->adj_term.p;
+    return mv;
+}
+
+DevMoves adjacency_only(mcx_ctx *h)
+{
+    DevMoves mv{};
+    mv.adj_off = h->adj_off.p;
+    mv.adj_term = h->adj_term.p;
+    mv.mass = h->mass64.p;
+    return mv;
+}
+
+void mt_seed(uint32_t *st, uint64_t seed)
+{
+    st[0] = (uint32_t)(seed & 0xffffffffu);
+    for (int i = 1; i < 624; i++) st[i] = 1812433253u * (st[i - 1] ^ (st[i - 1] >> 30)) + (uint32_t)i;
+    st[624] = 624;
+    st[625] = 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int mcx_create(int natom, int device, mcx_ctx **out)
+{
+    if (!out || natom <= 0) return MCX_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return MCX_ERR_HIP;
+    auto h = std::unique_ptr<mcx_ctx>(new mcx_ctx());
+    h->n = natom;
+    h->device = device;
+    int rc = guarded(h.get(), [&] {
+        HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        h->q.assign(natom, 0); h->eps.assign(natom, 0); h->sig.assign(natom, 0);
+        h->eps14.assign(natom, 0); h->sig14.assign(natom, 0); h->mass.assign(natom, 1.0);
+        h->frozen.assign(natom, 0);
+        h->resid.resize(natom);
+        for (int i = 0; i < natom; i++) h->resid[i] = i + 1;
+    });
+    if (rc != MCX_OK) return rc;
+    *out = h.release();
+    return MCX_OK;
+}
+
+void mcx_destroy(mcx_ctx *h)
+{
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    delete h;
+}
+
+const char *mcx_last_error(const mcx_ctx *h) { return h ? h->err.c_str() : "null handle"; }
+
+int mcx_set_box(mcx_ctx *h, int pbtype, double a, double b, double c)
+{
+    return guarded(h, [&] {
+        if (pbtype != MCX_PBC_NONE && pbtype != MCX_PBC_CUBIC && pbtype != MCX_PBC_ORBIC)
+            throw mcx_error(MCX_ERR_ARG, "unknown pbc type");
+        if (pbtype == MCX_PBC_CUBIC) { b = a; c = a; }
+        if (pbtype != MCX_PBC_NONE && !(a > 0 && b > 0 && c > 0)) throw mcx_error(MCX_ERR_ARG, "box lengths must be > 0");
+        h->pbtype = pbtype;
+        h->box[0] = a; h->box[1] = b; h->box[2] = c;
+        h->list_valid = false;
+        h->ntiles = 0;
+        upload_mirror(h);
+    });
+}
+
+int mcx_set_cut(mcx_ctx *h, int cutmode, double ctoff, double cton, double dcut, double temp)
+{
+    return guarded(h, [&] {
+        if (cutmode != MCX_CUT_FULL && cutmode != MCX_CUT_SWITCH) throw mcx_error(MCX_ERR_ARG, "unknown cut mode");
+        if (cutmode == MCX_CUT_SWITCH && !(ctoff > cton && cton >= 0 && dcut >= 0))
+            throw mcx_error(MCX_ERR_ARG, "switch needs ctoff > cton >= 0 and dcut >= 0");
+        h->cutmode = cutmode; h->ctoff = ctoff; h->cton = cton; h->dcut = dcut; h->temp = temp;
+        h->list_valid = false;
+    });
+}
+
+int mcx_set_atoms(mcx_ctx *h, const double *charge, const double *eps, const double *sig, const double *eps14,
+                  const double *sig14, const double *mass, const int *frozen, const int *resid)
+{
+    return guarded(h, [&] {
+        if (!charge || !eps || !sig) throw mcx_error(MCX_ERR_ARG, "charge, eps and sig are required");
+        const int n = h->n;
+        h->q.assign(charge, charge + n);
+        h->eps.assign(eps, eps + n);
+        h->sig.assign(sig, sig + n);
+        h->eps14.assign(eps14 ? eps14 : eps, (eps14 ? eps14 : eps) + n);
+        h->sig14.assign(sig14 ? sig14 : sig, (sig14 ? sig14 : sig) + n);
+        if (mass) h->mass.assign(mass, mass + n);
+        if (frozen) h->frozen.assign(frozen, frozen + n);
+        if (resid) h->resid.assign(resid, resid + n);
+        std::vector<float2> e(n), e14(n);
+        for (int i = 0; i < n; i++) {
+            e[i] = make_float2((float)h->eps[i], (float)h->sig[i]);
+            e14[i] = make_float2((float)h->eps14[i], (float)h->sig14[i]);
+        }
+        h->es.upload(e.data(), n, h->stream);
+        h->es14.upload(e14.data(), n, h->stream);
+        h->q64.upload(h->q.data(), n, h->stream);
+        h->eps64.upload(h->eps.data(), n, h->stream);
+        h->sig64.upload(h->sig.data(), n, h->stream);
+        h->eps1464.upload(h->eps14.data(), n, h->stream);
+        h->sig1464.upload(h->sig14.data(), n, h->stream);
+        h->mass64.upload(h->mass.data(), n, h->stream);
+        h->frozen_d.upload(h->frozen.data(), n, h->stream);
+        h->have_atoms = true;
+        upload_mirror(h);
+        HIPCHK(hipStreamSynchronize(h->stream));
+    });
+}
+
+static void check_idx(const mcx_ctx *h, int n, const int *const *arrs, int narr)
+{
+    for (int a = 0; a < narr; a++)
+        for (int k = 0; k < n; k++)
+            if (arrs[a][k] < 0 || arrs[a][k] >= h->n) throw mcx_error(MCX_ERR_ARG, "term atom index out of range");
+}
+
+int mcx_set_bonds(mcx_ctx *h, int n, const int *a, const int *b, const int *type, const double *k, const double *r0,
+                  const double *beta)
+{
+    return guarded(h, [&] {
+        const int *ar[2] = {a, b};
+        if (n > 0) check_idx(h, n, ar, 2);
+        auto &T = h->terms;
+        T.idx[kBond].clear(); T.type[kBond].clear(); T.par[kBond].clear();
+        for (int t = 0; t < n; t++) {
+            T.idx[kBond].push_back(make_int4(a[t], b[t], -1, -1));
+            T.type[kBond].push_back(type ? type[t] : 0);
+            T.par[kBond].push_back(make_double4(k[t], r0[t], beta ? beta[t] : 0.0, 0.0));
+        }
+        h->have_excl = false;
+    });
+}
+
+int mcx_set_ub(mcx_ctx *h, int n, const int *a, const int *b, const double *k, const double *r0)
+{
+    return guarded(h, [&] {
+        const int *ar[2] = {a, b};
+        if (n > 0) check_idx(h, n, ar, 2);
+        auto &T = h->terms;
+        T.idx[kUB].clear(); T.type[kUB].clear(); T.par[kUB].clear();
+        for (int t = 0; t < n; t++) {
+            T.idx[kUB].push_back(make_int4(a[t], b[t], -1, -1));
+            T.type[kUB].push_back(0);
+            T.par[kUB].push_back(make_double4(k[t], r0[t], 0.0, 0.0));
+        }
+    });
+}
+
+int mcx_set_angles(mcx_ctx *h, int n, const int *a, const int *b, const int *c, const double *k, const double *theta0)
+{
+    return guarded(h, [&] {
+        const int *ar[3] = {a, b, c};
+        if (n > 0) check_idx(h, n, ar, 3);
+        auto &T = h->terms;
+        T.idx[kAngle].clear(); T.type[kAngle].clear(); T.par[kAngle].clear();
+        for (int t = 0; t < n; t++) {
+            T.idx[kAngle].push_back(make_int4(a[t], b[t], c[t], -1));
+            T.type[kAngle].push_back(0);
+            T.par[kAngle].push_back(make_double4(k[t], theta0[t], 0.0, 0.0));
+        }
+        h->have_excl = false;
+    });
+}
+
+static int set_torsions(mcx_ctx *h, int kind, int n, const int *a, const int *b, const int *c, const int *d,
+                        const int *type, const double *k, const double *phi0, const double *mult)
+{
+    return guarded(h, [&] {
+        const int *ar[4] = {a, b, c, d};
+        if (n > 0) check_idx(h, n, ar, 4);
+        auto &T = h->terms;
+        T.idx[kind].clear(); T.type[kind].clear(); T.par[kind].clear();
+        for (int t = 0; t < n; t++) {
+            T.idx[kind].push_back(make_int4(a[t], b[t], c[t], d[t]));
+            T.type[kind].push_back(type ? type[t] : 1);
+            T.par[kind].push_back(make_double4(k[t], phi0[t], mult ? mult[t] : 0.0, 0.0));
+        }
+        h->have_excl = false;
+    });
+}
+
+int mcx_set_dihedrals(mcx_ctx *h, int n, const int *a, const int *b, const int *c, const int *d, const int *type,
+                      const double *k, const double *phi0, const double *mult)
+{
+    return set_torsions(h, kDihe, n, a, b, c, d, type, k, phi0, mult);
+}
+
+int mcx_set_impropers(mcx_ctx *h, int n, const int *a, const int *b, const int *c, const int *d, const int *type,
+                      const double *k, const double *phi0, const double *mult)
+{
+    return set_torsions(h, kImpr, n, a, b, c, d, type, k, phi0, mult);
+}
+
+int mcx_build_exclusions(mcx_ctx *h)
+{
+    return guarded(h, [&] {
+        build_exclusions_host(h);
+        upload_terms(h);
+        HIPCHK(hipStreamSynchronize(h->stream));
+        h->have_excl = true;
+        h->list_valid = false;
+    });
+}
+
+int mcx_get_exclusions(mcx_ctx *h, int *exclPair, int *idx, long long cap, long long *nidx, int *pairs14, int cap14,
+                       int *npair14)
+{
+    return guarded(h, [&] {
+        if (!h->have_excl) throw mcx_error(MCX_ERR_STATE, "exclusions not built");
+        if (exclPair) std::copy(h->excl_pair.begin(), h->excl_pair.end(), exclPair);
+        if (nidx) *nidx = (long long)h->excl_idx.size();
+        if (idx) for (long long k = 0; k < (long long)h->excl_idx.size() && k < cap; k++) idx[k] = h->excl_idx[k];
+        if (npair14) *npair14 = (int)h->pairs14.size() / 2;
+        if (pairs14) for (int k = 0; k < (int)h->pairs14.size() && k < cap14; k++) pairs14[k] = h->pairs14[k];
+    });
+}
+
+int mcx_set_coords(mcx_ctx *h, const double *x, const double *y, const double *z)
+{
+    return guarded(h, [&] {
+        if (!x || !y || !z) throw mcx_error(MCX_ERR_ARG, "null coordinate array");
+        upload_coords(h, x, y, z);
+    });
+}
+
+int mcx_get_coords(mcx_ctx *h, double *x, double *y, double *z)
+{
+    return guarded(h, [&] {
+        if (!h->have_coords) throw mcx_error(MCX_ERR_STATE, "no coordinates");
+        HIPCHK(hipMemcpyAsync(x, h->x.p, h->n * 8, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(y, h->y.p, h->n * 8, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(z, h->z.p, h->n * 8, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    });
+}
+
+int mcx_update_list(mcx_ctx *h, long long *npairs)
+{
+    return guarded(h, [&] {
+        long long np = 0;
+        if (h->cutmode != MCX_CUT_FULL) np = build_list(h);
+        if (npairs) *npairs = np;
+    });
+}
+
+int mcx_get_list(mcx_ctx *h, int *neighPair, int *flat, long long cap)
+{
+    return guarded(h, [&] {
+        if (!h->list_valid) throw mcx_error(MCX_ERR_STATE, "list not built");
+        const int n = h->n;
+        std::vector<int> c(n), hf(n), nb((size_t)n * h->cap);
+        HIPCHK(hipMemcpy(c.data(), h->cnt.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(hf.data(), h->half.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(nb.data(), h->nbr.p, sizeof(int) * nb.size(), hipMemcpyDeviceToHost));
+        long long o = 0;
+        for (int i = 0; i < n; i++) {
+            if (neighPair) neighPair[i] = c[i] - hf[i];
+            for (int k = hf[i]; k < c[i]; k++) {
+                if (flat && o < cap) flat[o] = nb[(size_t)i * h->cap + k];
+                o++;
+            }
+        }
+    });
+}
+
+int mcx_energy(mcx_ctx *h, double ener[10])
+{
+    return guarded(h, [&] {
+        double r[9];
+        energy_device(h, r);
+        assemble(r, ener);
+    });
+}
+
+int mcx_delta_energy(mcx_ctx *h, int first, int last, const double *nx, const double *ny, const double *nz, double *de)
+{
+    return guarded(h, [&] {
+        if (first < 0 || last >= h->n || last < first) throw mcx_error(MCX_ERR_ARG, "bad atom range");
+        if (last - first + 1 > kMaxMoveAtoms) throw mcx_error(MCX_ERR_UNSUPPORTED, "at most 64 moved atoms");
+        if (!h->have_excl || !h->have_coords) throw mcx_error(MCX_ERR_STATE, "system not set up");
+        if (h->cutmode != MCX_CUT_FULL && !h->list_valid) build_list(h);
+        const int nm = last - first + 1;
+        std::vector<double> buf(3 * nm);
+        for (int a = 0; a < nm; a++) { buf[3 * a] = nx[a]; buf[3 * a + 1] = ny[a]; buf[3 * a + 2] = nz[a]; }
+        h->newxyz.upload(buf.data(), buf.size(), h->stream);
+        h->scalar.alloc(4);
+        launch_delta_energy(h->sys(), h->dterms(), adjacency_only(h), first, last, h->newxyz.p, h->scalar.p, h->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(de, h->scalar.p, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    });
+}
+
+int mcx_mc_run(mcx_ctx *h, const mcx_move_type *types, int ntypes, int nsteps, uint64_t seed, int rng_mode,
+               int list_every, mcx_mc_stats *stats, unsigned char *acc_out, double *de_out, int *mvtyp_out,
+               int *mvinst_out)
+{
+    return guarded(h, [&] {
+        if (nsteps < 0) throw mcx_error(MCX_ERR_ARG, "nsteps < 0");
+        if (!h->have_excl || !h->have_coords || !h->have_atoms) throw mcx_error(MCX_ERR_STATE, "system not set up");
+        if (h->cutmode != MCX_CUT_FULL && !h->list_valid) build_list(h);
+        DevMoves mv = build_moves(h, types, ntypes);
+        // device-side run state
+        DBuf<double> etot, dmax;
+        DBuf<uint32_t> mts;
+        DBuf<long long> trials, accepted;
+        DBuf<int> acc_tmp;
+        DBuf<unsigned char> acc_d;
+        DBuf<double> de_d;
+        DBuf<int> typ_d, inst_d;
+        struct Cleanup {
+            std::vector<std::function<void()>> f;
+            ~Cleanup() { for (auto &g : f) g(); }
+        } cleanup;
+        cleanup.f.push_back([&] { etot.release(); dmax.release(); mts.release(); trials.release(); accepted.release();
+                                  acc_tmp.release(); acc_d.release(); de_d.release(); typ_d.release(); inst_d.release(); });
+        double ener[10], r9[9];
+        energy_device(h, r9);
+        assemble(r9, ener);
+        etot.upload(&ener[0], 1, h->stream);
+        std::vector<double> dm(ntypes);
+        for (int t = 0; t < ntypes; t++) dm[t] = types[t].dmax;
+        dmax.upload(dm.data(), ntypes, h->stream);
+        std::vector<uint32_t> st(626);
+        mt_seed(st.data(), seed);
+        if (rng_mode != MCX_MC_REPLAY) { st[624] = 0; st[625] = 0; }
+        mts.upload(st.data(), st.size(), h->stream);
+        std::vector<long long> z8(ntypes, 0);
+        trials.upload(z8.data(), ntypes, h->stream);
+        accepted.upload(z8.data(), ntypes, h->stream);
+        std::vector<int> zi(ntypes, 0);
+        acc_tmp.upload(zi.data(), ntypes, h->stream);
+        if (acc_out) acc_d.alloc(std::max(nsteps, 1));
+        if (de_out) de_d.alloc(std::max(nsteps, 1));
+        if (mvtyp_out) typ_d.alloc(std::max(nsteps, 1));
+        if (mvinst_out) inst_d.alloc(std::max(nsteps, 1));
+        MCRunArgs a{};
+        a.rng_mode = rng_mode == MCX_MC_REPLAY ? 1 : 0;
+        a.seed = seed;
+        a.beta = 1.0 / ((MCX_RBOLTZUI / MCX_KCALTOIU) * h->temp);
+        a.etot = etot.p; a.mt_state = mts.p; a.dmax = dmax.p; a.trials = trials.p; a.accepted = accepted.p;
+        a.acc_tmp = acc_tmp.p;
+        a.acc_out = acc_out ? acc_d.p : nullptr;
+        a.de_out = de_out ? de_d.p : nullptr;
+        a.mvtyp_out = mvtyp_out ? typ_d.p : nullptr;
+        a.mvinst_out = mvinst_out ? inst_d.p : nullptr;
+        HIPCHK(hipStreamSynchronize(h->stream));
+        const auto t0 = std::chrono::steady_clock::now();
+        long long nlist = 0;
+        int done = 0;
+        const int every = (list_every > 0 && h->cutmode != MCX_CUT_FULL) ? list_every : nsteps;
+        while (done < nsteps) {
+            int chunk = every - (done % std::max(every, 1));
+            if (chunk > nsteps - done) chunk = nsteps - done;
+            a.nsteps = chunk;
+            a.step0 = done;
+            launch_mc_chunk(h->sys(), h->dterms(), mv, a, h->stream);
+            HIPCHK(hipGetLastError());
+            done += chunk;
+            if (list_every > 0 && h->cutmode != MCX_CUT_FULL && done % list_every == 0) {
+                rebuild_list_fast(h);
+                nlist++;
+            }
+        }
+        HIPCHK(hipStreamSynchronize(h->stream));
+        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (stats) {
+            std::memset(stats, 0, sizeof(*stats));
+            HIPCHK(hipMemcpy(&stats->etot, etot.p, sizeof(double), hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(stats->trials, trials.p, ntypes * sizeof(long long), hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(stats->accepted, accepted.p, ntypes * sizeof(long long), hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(stats->dmax, dmax.p, ntypes * sizeof(double), hipMemcpyDeviceToHost));
+            stats->list_updates = nlist;
+            stats->seconds = secs;
+        }
+        sync_host_coords(h);
+        if (acc_out && nsteps) HIPCHK(hipMemcpy(acc_out, acc_d.p, nsteps, hipMemcpyDeviceToHost));
+        if (de_out && nsteps) HIPCHK(hipMemcpy(de_out, de_d.p, nsteps * sizeof(double), hipMemcpyDeviceToHost));
+        if (mvtyp_out && nsteps) HIPCHK(hipMemcpy(mvtyp_out, typ_d.p, nsteps * sizeof(int), hipMemcpyDeviceToHost));
+        if (mvinst_out && nsteps) HIPCHK(hipMemcpy(mvinst_out, inst_d.p, nsteps * sizeof(int), hipMemcpyDeviceToHost));
+    });
+}
+
+int mcx_nonbonded_grad(mcx_ctx *h, double *gx, double *gy, double *gz, double *geps, double *gsig, double *gq)
+{
+    return guarded(h, [&] {
+        if (h->cutmode != MCX_CUT_SWITCH) throw mcx_error(MCX_ERR_UNSUPPORTED, "gradients need SWITCH mode");
+        if (!h->list_valid) build_list(h);
+        const int n = h->n;
+        DBuf<double> g;
+        g.alloc((size_t)6 * n);
+        launch_nonbonded_grad(h->sys(), g.p, h->stream);
+        HIPCHK(hipGetLastError());
+        std::vector<double> hg((size_t)6 * n);
+        HIPCHK(hipMemcpyAsync(hg.data(), g.p, hg.size() * 8, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        g.release();
+        double *outs[6] = {gx, gy, gz, geps, gsig, gq};
+        for (int c = 0; c < 6; c++)
+            if (outs[c]) std::copy(hg.begin() + (size_t)c * n, hg.begin() + (size_t)(c + 1) * n, outs[c]);
+    });
+}
+
+int mcx_device_coords(mcx_ctx *h, int which, void **ptr)
+{
+    return guarded(h, [&] {
+        if (!ptr || which < 0 || which > 2) throw mcx_error(MCX_ERR_ARG, "which must be 0..2");
+        *ptr = which == 0 ? (void *)h->x.p : which == 1 ? (void *)h->y.p : (void *)h->z.p;
+    });
+}
+
+int mcx_time_kernel(mcx_ctx *h, int which, int reps, double *ms)
+{
+    return guarded(h, [&] {
+        if (reps < 1) throw mcx_error(MCX_ERR_ARG, "reps < 1");
+        DevSys s = h->sys();
+        DevTerms t = h->dterms();
+        if (which == 1 || which == 2) { if (!h->list_valid) build_list(h); s = h->sys(); }
+        if (which == 0 && h->ntiles == 0) setup_tiles(h);
+        const int nbs = switch_energy_blocks(h->n);
+        h->partial.alloc((size_t)std::max({h->ntiles * 2, nbs * 2, bonded_blocks(t) * 7, 16}));
+        auto launch = [&] {
+            switch (which) {
+            case 0: launch_full_energy(s, h->tiles.p, h->ntiles, 256, h->partial.p, h->stream); break;
+            case 1: launch_switch_energy(s, h->partial.p, nbs, h->stream); break;
+            case 2: launch_list_build(s, h->grid, h->stream); break;
+            case 3: launch_bonded(s, t, h->partial.p, bonded_blocks(t), h->stream); break;
+            default: throw mcx_error(MCX_ERR_ARG, "unknown kernel id");
+            }
+        };
+        launch();   // warm-up
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, h->stream));
+        for (int r = 0; r < reps; r++) launch();
+        HIPCHK(hipEventRecord(e1, h->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float t_ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&t_ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms = t_ms / reps;
+    });
+}
+
+}  // extern "C"

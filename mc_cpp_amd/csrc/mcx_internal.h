@@ -1,0 +1,147 @@
+// mcx_internal.h -- shared definitions of the gfx950 energy backend.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+// Internal units and constants (include/Constants.hpp:29-42 of mc_cpp).
+#define MCX_KCALTOIU 418.4
+#define MCX_RBOLTZUI 0.83144621
+#define MCX_CHGCHARMM 332.0716
+#define MCX_PI 3.141592653589793
+#define MCX_TWOPI 6.283185307179586
+
+// Algorithmic work per unit, frozen here so that the benchmark's roofline
+// figures use one set of constants (FMA = 2 FLOP, rsqrt = 1, compares and
+// rounding = 0):
+//   switched list pair : 17 FLOP per list entry + 23 per in-cutoff entry
+//   full O(N^2) pair   : 32 FLOP per pair
+//   list build bytes   : 24*N + 4*(N+1) + 4*P
+#define MCX_FLOP_LIST_ENTRY 17.0
+#define MCX_FLOP_LIST_INCUT 23.0
+#define MCX_FLOP_FULL_PAIR 32.0
+
+#define MCX_MV_TRN_DEV 1   // move type codes as in include/Move.hpp:33-38
+#define MCX_MV_ROT_DEV 2
+
+constexpr int kMaxMoveAtoms = 64;   // atoms per move instance on the device path
+constexpr int kMaxMoveTypes = 8;
+
+// Everything a kernel needs about one system, passed by value.
+struct DevSys {
+    int n;
+    int pbtype;
+    double L[3], rL[3];
+    float Lf[3], rLf[3];
+    double ctoff2, cton2, switch2, cutnb2;
+    int cutmode;
+    // coordinates: FP64 master copy + a 32-bit fixed-point mirror of the
+    // fractional coordinates (x/L_k mod 1 scaled by 2^32) with the charge's
+    // FP32 bits in .w.  A wrapping int32 difference of two mirrors is the
+    // minimum-image displacement in units of L_k/2^32, exactly.
+    double *x, *y, *z;
+    int4 *ipos;
+    double unit[3];      // L_k / 2^32 (1 when pbtype == NONE)
+    double iunit;        // 2^32 / L for cubic boxes (kernel length unit), else 1
+    float2 *es;          // (eps, sig) FP32, sig in A
+    float2 *esk;         // (eps, sig) FP32, sig in kernel length units
+    float2 *es14;        // (eps14, sig14)
+    const double *q64, *eps64, *sig64, *eps1464, *sig1464;
+    const int *frozen;
+    // exclusion rows (visible entries, CSR)
+    const int *excl_off, *excl_idx;
+    // neighbour list: full rows (all j != i, ascending) in ELL layout
+    int cap;
+    int *nbr;            // [n*cap]
+    int *cnt;            // [n]
+    int *half;           // [n] index of the first j > i in the row
+    int *overflow;       // [1]
+};
+
+// Bonded terms in a tagged work list for the fused bonded kernel.
+enum TermKind : int {
+    kBond = 0, kUB = 1, kAngle = 2, kDihe = 3, kImpr = 4, kPair14 = 5, kExclPair = 6, kNumKinds = 7
+};
+
+struct DevTerms {
+    int nterm[kNumKinds];
+    int start[kNumKinds + 1];   // prefix of nterm: global work index ranges
+    const int4 *idx[kNumKinds]; // atom indices (unused lanes = -1)
+    const int *type[kNumKinds];
+    const double4 *par[kNumKinds]; // k, r0/theta0/phi0, beta/mult, unused
+};
+
+struct DevMoves {
+    int ntypes;
+    int type[kMaxMoveTypes];
+    int ninst[kMaxMoveTypes];
+    int off[kMaxMoveTypes];
+    double dmax0[kMaxMoveTypes];
+    double target[kMaxMoveTypes];
+    int each[kMaxMoveTypes];
+    const int *first, *last, *pivot;   // flat instance tables
+    const double *mass;                // for centre-of-mass pivots
+    // per-atom adjacency of bonded terms (CSR over atoms -> global term ids)
+    const int *adj_off, *adj_term;
+};
+
+struct HostTerms {
+    std::vector<int4> idx[kNumKinds];
+    std::vector<int> type[kNumKinds];
+    std::vector<double4> par[kNumKinds];
+};
+
+#define HIPCHK(expr)                                                                   \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess) throw mcx_error(MCX_ERR_HIP, std::string(#expr) + ": " + \
+                                              hipGetErrorString(_e));                  \
+    } while (0)
+
+struct mcx_error {
+    int code;
+    std::string msg;
+    mcx_error(int c, std::string m) : code(c), msg(std::move(m)) {}
+};
+
+// ---- launchers implemented in the .hip files ----
+// nonbonded.hip
+void launch_full_energy(const DevSys &s, const int2 *tiles, int ntiles, int tile, double *partial,
+                        hipStream_t st);
+void launch_switch_energy(const DevSys &s, double *partial, int nblocks, hipStream_t st);
+void launch_reduce(const double *partial, int nblocks, int ncomp, double *out, hipStream_t st);
+int switch_energy_blocks(int n);
+// list.hip
+struct CellGrid {
+    int nc[3];
+    int ncell;
+    int reach[3];            // cells searched in each direction
+    int *cell_of, *cell_start, *cell_count, *cell_atoms, *cursor;
+};
+void launch_list_build(const DevSys &s, const CellGrid &g, hipStream_t st);
+size_t list_build_lds_bytes(int n);
+// bonded.hip
+void launch_bonded(const DevSys &s, const DevTerms &t, double *partial, int nblocks, hipStream_t st);
+int bonded_blocks(const DevTerms &t);
+// mc.hip
+struct DevMCState;
+void launch_delta_energy(const DevSys &s, const DevTerms &t, const DevMoves &mv, int first, int last,
+                         const double *newxyz, double *de, hipStream_t st);
+struct MCRunArgs {
+    int nsteps, step0, rng_mode;
+    uint64_t seed;
+    double beta;
+    double *etot;          // device scalar
+    uint32_t *mt_state;    // [626] device: MT19937 state + index, or the Philox counter in [624..625]
+    double *dmax;          // [ntypes] device
+    long long *trials, *accepted;   // [ntypes] device
+    int *acc_tmp;          // [ntypes]
+    unsigned char *acc_out; double *de_out; int *mvtyp_out; int *mvinst_out;  // device or null
+};
+void launch_mc_chunk(const DevSys &s, const DevTerms &t, const DevMoves &mv, const MCRunArgs &a,
+                     hipStream_t st);
+// adjoint.hip
+void launch_nonbonded_grad(const DevSys &s, double *g6, hipStream_t st);

@@ -1,0 +1,208 @@
+// nonbonded.hip -- LJ + Coulomb pair energies on gfx950.
+//
+//  * k_full_energy: all pairs i<j with minimum image and no cutoff
+//    (computeNonBonded_full, src/FField_MDBAS.cpp:194-265).  Triangular
+//    256x256 tiles; one i atom per lane, the j atom uniform across the
+//    wavefront so that its data arrives through scalar loads and LJ is
+//    skipped by a scalar branch when eps_j = 0.  Excluded pairs are included
+//    here and subtracted by the fused bonded kernel, which keeps this loop
+//    free of per-pair branches.
+//  * k_switch_energy: the switched sum over the Verlet list
+//    (computeNonBonded_switch, :336-424).  One wavefront per atom row of
+//    the sorted full list, reading the j > i suffix.
+//
+// Pair arithmetic: the displacement is the wrapping int32 difference of the
+// fixed-point fractional coordinates (the exact minimum image, no rint),
+// converted to FP64; r^2 in FP64; 1/r from the FP32 v_rsq estimate refined
+// by one FP64 Newton step; the Coulomb sum in FP64.  The LJ part uses the
+// FP32 estimate of 1/r and FP32 arithmetic, folded into FP64 every 8 pairs.
+// FP64 FMAs issue at the FP32 rate on this GPU, so the Coulomb refinement
+// costs a few issue slots and buys the 1e-6 relative total that water's
+// 700-fold Coulomb cancellation demands.
+#include "terms.h"
+
+namespace {
+
+constexpr int kTile = 256;
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block-wide FP64 sum of two values; result valid in thread 0.
+template <int NT>
+__device__ __forceinline__ void block_sum2(double &a, double &b)
+{
+    __shared__ double sa[NT / 64], sb[NT / 64];
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) { sa[w] = a; sb[w] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ta = 0.0, tb = 0.0;
+        for (int k = 0; k < NT / 64; k++) { ta += sa[k]; tb += sb[k]; }
+        a = ta; b = tb;
+    }
+}
+
+template <int PB>
+__device__ __forceinline__ void disp(const DevSys &s, const int4 &pi, const int4 &pj, double xi, double yi,
+                                     double zi, int j, double &dx, double &dy, double &dz)
+{
+    mcx_disp_k<PB>(s, pi, pj, xi, yi, zi, j, dx, dy, dz);
+}
+
+template <int PB, bool DIAG>
+__device__ __forceinline__ void full_tile(const DevSys &s, int i, int j0, int j1, const int4 pi, double xi, double yi,
+                                          double zi, float si, double &accC, double &accV)
+{
+    const int4 *__restrict__ ipos = s.ipos;
+    const float2 *__restrict__ esk = s.esk;
+    for (int j = j0; j < j1; j++) {
+        const int4 pj = ipos[j];
+        const float2 ej = esk[j];
+        double ri;
+        float lj;
+        mcx_tile_pair<PB>(s, pi, pj, xi, yi, zi, j, si + ej.y, ri, lj);
+        if (DIAG && !(j > i)) continue;
+        accC = fma((double)__int_as_float(pj.w), ri, accC);
+        if (ej.x != 0.f) accV = fma((double)ej.x, (double)lj, accV);
+    }
+}
+
+template <int PB>
+__global__ __launch_bounds__(kTile) void k_full_energy(DevSys s, const int2 *__restrict__ tiles,
+                                                        double *__restrict__ partial)
+{
+    const int2 tl = tiles[blockIdx.x];
+    const int i = tl.x * kTile + threadIdx.x;
+    const bool valid = i < s.n;
+    const int ii = valid ? i : 0;
+    const int4 pi = s.ipos[ii];
+    const float2 ei = s.esk[ii];
+    const double xi = PB == 0 ? s.x[ii] : 0.0, yi = PB == 0 ? s.y[ii] : 0.0, zi = PB == 0 ? s.z[ii] : 0.0;
+    const int j0 = tl.y * kTile;
+    const int j1 = min(j0 + kTile, s.n);
+    double accC = 0.0, accV = 0.0;
+    if (tl.x == tl.y) full_tile<PB, true>(s, i, j0, j1, pi, xi, yi, zi, ei.y, accC, accV);
+    else full_tile<PB, false>(s, i, j0, j1, pi, xi, yi, zi, ei.y, accC, accV);
+    double ec = valid ? (double)__int_as_float(pi.w) * accC * s.iunit : 0.0;
+    double ev = valid ? (double)ei.x * accV : 0.0;
+    block_sum2<kTile>(ec, ev);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = ec;
+        partial[2 * blockIdx.x + 1] = ev;
+    }
+}
+
+// ---------------------------------------------------------------------------
+
+constexpr int kSwWaves = 4;
+
+template <int PB>
+__global__ __launch_bounds__(64 * kSwWaves) void k_switch_energy(DevSys s, double *__restrict__ partial)
+{
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    double ec = 0.0, ev = 0.0;
+    // kernel length unit -> A^2
+    const double u2 = PB == 1 ? s.unit[0] * s.unit[0] : 1.0;
+    for (int i = blockIdx.x * kSwWaves + w; i < s.n; i += gridDim.x * kSwWaves) {
+        const int4 pi = s.ipos[i];
+        const float2 ei = s.esk[i];
+        const double xi = PB == 0 ? s.x[i] : 0.0, yi = PB == 0 ? s.y[i] : 0.0, zi = PB == 0 ? s.z[i] : 0.0;
+        const int *__restrict__ row = s.nbr + (size_t)i * s.cap;
+        const int k1 = s.cnt[i];
+        double c = 0.0, v = 0.0;
+        for (int k = s.half[i] + lane; k < k1; k += 64) {
+            const int j = row[k];
+            const int4 pj = s.ipos[j];
+            const float2 ej = s.esk[j];
+            double dx, dy, dz;
+            disp<PB>(s, pi, pj, xi, yi, zi, j, dx, dy, dz);
+            const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+            const double r2A = r2 * u2;
+            if (r2A <= s.ctoff2) {
+                float ri32;
+                const double ri = mcx_rsqrt(r2, ri32);
+                const float uu = (ei.y + ej.y) * ri32;
+                const float t = uu * uu;
+                const float t3 = t * t * t;
+                const double vj = (double)(ej.x * fmaf(t3, t3, -t3));
+                double sw = 1.0;
+                if (r2A > s.cton2) {
+                    const double a = s.ctoff2 - r2A;
+                    sw = a * a * (s.ctoff2 + 2.0 * r2A - 3.0 * s.cton2) * s.switch2;
+                }
+                c = fma(sw * (double)__int_as_float(pj.w), ri, c);
+                v = fma(sw, vj, v);
+            }
+        }
+        c = wave_sum(c);
+        v = wave_sum(v);
+        ec += (double)__int_as_float(pi.w) * c;
+        ev += (double)ei.x * v;
+    }
+    // lanes hold identical wave sums; keep lane 0's copy only
+    if (lane != 0) { ec = 0.0; ev = 0.0; }
+    ec *= s.iunit;
+    block_sum2<64 * kSwWaves>(ec, ev);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = ec;
+        partial[2 * blockIdx.x + 1] = ev;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ partial, int nblocks, int ncomp,
+                                                 double *__restrict__ out)
+{
+    // thread t sums blocks t, t+256, ... of component c in order, then a
+    // fixed-shape tree: the result does not depend on scheduling.
+    __shared__ double sm[256];
+    for (int c = 0; c < ncomp; c++) {
+        double v = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += 256) v += partial[(size_t)b * ncomp + c];
+        sm[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[c] = sm[0];
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+void launch_full_energy(const DevSys &s, const int2 *tiles, int ntiles, int tile, double *partial, hipStream_t st)
+{
+    (void)tile;
+    const int pb = s.pbtype == 0 ? 0 : (s.pbtype == 1 ? 1 : 2);
+    if (pb == 1) hipLaunchKernelGGL(k_full_energy<1>, dim3(ntiles), dim3(kTile), 0, st, s, tiles, partial);
+    else if (pb == 2) hipLaunchKernelGGL(k_full_energy<2>, dim3(ntiles), dim3(kTile), 0, st, s, tiles, partial);
+    else hipLaunchKernelGGL(k_full_energy<0>, dim3(ntiles), dim3(kTile), 0, st, s, tiles, partial);
+}
+
+int switch_energy_blocks(int n)
+{
+    int b = (n + kSwWaves - 1) / kSwWaves;
+    return b < 1 ? 1 : b;
+}
+
+void launch_switch_energy(const DevSys &s, double *partial, int nblocks, hipStream_t st)
+{
+    const int pb = s.pbtype == 0 ? 0 : (s.pbtype == 1 ? 1 : 2);
+    if (pb == 1) hipLaunchKernelGGL(k_switch_energy<1>, dim3(nblocks), dim3(64 * kSwWaves), 0, st, s, partial);
+    else if (pb == 2) hipLaunchKernelGGL(k_switch_energy<2>, dim3(nblocks), dim3(64 * kSwWaves), 0, st, s, partial);
+    else hipLaunchKernelGGL(k_switch_energy<0>, dim3(nblocks), dim3(64 * kSwWaves), 0, st, s, partial);
+}
+
+void launch_reduce(const double *partial, int nblocks, int ncomp, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, st, partial, nblocks, ncomp, out);
+}

@@ -1,0 +1,337 @@
+"""ctypes binding of libmcx.so (include/mcx.h) and a host-side mirror of the
+reference's energy plugin interface:
+
+  * FField       -- getE(), getE(ener[10]) as getE10(), askListUpdate(st)
+                    (include/FField.hpp:95-98)
+  * List_nonBonded accessors -- getNeighPair/getNeighList/getExclList/...
+                    (include/List_nonBonded.hpp:50-59)
+  * MC_metropolis -- run() on the device (src/MC_metropolis.cpp:45-204)
+
+The HIP library is the only compute path: if libmcx.so is missing or no GPU
+is visible, construction raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import systems as S
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libmcx.so")
+
+MCX_OK = 0
+ERRORS = {1: "MCX_ERR_ARG", 2: "MCX_ERR_HIP", 3: "MCX_ERR_LIST_OVERFLOW", 4: "MCX_ERR_BOX_TOO_SMALL",
+          5: "MCX_ERR_STATE", 6: "MCX_ERR_UNSUPPORTED"}
+MC_REPLAY, MC_PHILOX = 1, 0
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class MCXError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (ERRORS.get(code, code), msg))
+        self.code = code
+
+
+class MoveTypeC(C.Structure):
+    _fields_ = [("type", C.c_int), ("mode", C.c_int), ("dmax", C.c_double), ("target", C.c_double),
+                ("each", C.c_int), ("sele", _ip)]
+
+
+class MCStatsC(C.Structure):
+    _fields_ = [("etot", C.c_double), ("trials", C.c_longlong * 8), ("accepted", C.c_longlong * 8),
+                ("dmax", C.c_double * 8), ("list_updates", C.c_longlong), ("seconds", C.c_double)]
+
+
+# every symbol include/mcx.h declares, with its argument types
+SIGNATURES = {
+    "mcx_create": ([C.c_int, C.c_int, C.POINTER(C.c_void_p)], C.c_int),
+    "mcx_destroy": ([C.c_void_p], None),
+    "mcx_last_error": ([C.c_void_p], C.c_char_p),
+    "mcx_set_box": ([C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double], C.c_int),
+    "mcx_set_cut": ([C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double], C.c_int),
+    "mcx_set_atoms": ([C.c_void_p] + [_dp] * 6 + [_ip, _ip], C.c_int),
+    "mcx_set_bonds": ([C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, _dp, _dp], C.c_int),
+    "mcx_set_ub": ([C.c_void_p, C.c_int, _ip, _ip, _dp, _dp], C.c_int),
+    "mcx_set_angles": ([C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, _dp], C.c_int),
+    "mcx_set_dihedrals": ([C.c_void_p, C.c_int, _ip, _ip, _ip, _ip, _ip, _dp, _dp, _dp], C.c_int),
+    "mcx_set_impropers": ([C.c_void_p, C.c_int, _ip, _ip, _ip, _ip, _ip, _dp, _dp, _dp], C.c_int),
+    "mcx_build_exclusions": ([C.c_void_p], C.c_int),
+    "mcx_get_exclusions": ([C.c_void_p, _ip, _ip, C.c_longlong, C.POINTER(C.c_longlong), _ip, C.c_int, _ip],
+                           C.c_int),
+    "mcx_set_coords": ([C.c_void_p, _dp, _dp, _dp], C.c_int),
+    "mcx_get_coords": ([C.c_void_p, _dp, _dp, _dp], C.c_int),
+    "mcx_update_list": ([C.c_void_p, C.POINTER(C.c_longlong)], C.c_int),
+    "mcx_get_list": ([C.c_void_p, _ip, _ip, C.c_longlong], C.c_int),
+    "mcx_energy": ([C.c_void_p, _dp], C.c_int),
+    "mcx_delta_energy": ([C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp], C.c_int),
+    "mcx_mc_run": ([C.c_void_p, C.POINTER(MoveTypeC), C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int,
+                    C.POINTER(MCStatsC), C.POINTER(C.c_ubyte), _dp, _ip, _ip], C.c_int),
+    "mcx_nonbonded_grad": ([C.c_void_p] + [_dp] * 6, C.c_int),
+    "mcx_device_coords": ([C.c_void_p, C.c_int, C.POINTER(C.c_void_p)], C.c_int),
+    "mcx_time_kernel": ([C.c_void_p, C.c_int, C.c_int, _dp], C.c_int),
+}
+
+_lib = None
+
+
+def load_library(path: str = LIBPATH):
+    """Loads libmcx.so and binds every declared entry point (raises if absent)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise RuntimeError("libmcx.so not built (run __graft_entry__.build()); no fallback path exists")
+        L = C.CDLL(path)
+        for name, (args, res) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class Handle:
+    """One mcx_ctx: a system resident on one GPU."""
+
+    def __init__(self, natom: int, device: int = 0):
+        self.L = load_library()
+        h = C.c_void_p()
+        rc = self.L.mcx_create(int(natom), int(device), C.byref(h))
+        if rc != MCX_OK:
+            raise MCXError(rc, "mcx_create failed (no usable HIP device %d?)" % device)
+        self.h = h
+        self.n = natom
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.mcx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != MCX_OK:
+            raise MCXError(rc, self.L.mcx_last_error(self.h).decode())
+
+    # ---- set-up ----
+    def set_box(self, pbtype, a, b=None, c=None):
+        b = a if b is None else b
+        c = a if c is None else c
+        self._chk(self.L.mcx_set_box(self.h, int(pbtype), float(a), float(b), float(c)))
+
+    def set_cut(self, cutmode, ctoff=12.0, cton=10.0, dcut=2.0, temp=300.0):
+        self._chk(self.L.mcx_set_cut(self.h, int(cutmode), float(ctoff), float(cton), float(dcut), float(temp)))
+
+    def set_atoms(self, charge, eps, sig, eps14=None, sig14=None, mass=None, frozen=None, resid=None):
+        arrs = [_d(charge), _d(eps), _d(sig), None if eps14 is None else _d(eps14),
+                None if sig14 is None else _d(sig14), None if mass is None else _d(mass)]
+        fr = None if frozen is None else _i(frozen)
+        rs = None if resid is None else _i(resid)
+        self._chk(self.L.mcx_set_atoms(self.h, *[_p(a, C.c_double) for a in arrs], _p(fr, C.c_int), _p(rs, C.c_int)))
+
+    def set_terms(self, s: S.System):
+        b, u, a, d, m = s.bonds, s.ub, s.angles, s.dihedrals, s.impropers
+        keep = []
+
+        def I(x):
+            x = _i(x); keep.append(x); return _p(x, C.c_int)
+
+        def D(x):
+            x = _d(x); keep.append(x); return _p(x, C.c_double)
+
+        self._chk(self.L.mcx_set_bonds(self.h, len(b["a"]), I(b["a"]), I(b["b"]), I(b["type"]), D(b["k"]),
+                                       D(b["r0"]), D(b["beta"])))
+        self._chk(self.L.mcx_set_ub(self.h, len(u["a"]), I(u["a"]), I(u["b"]), D(u["k"]), D(u["r0"])))
+        self._chk(self.L.mcx_set_angles(self.h, len(a["a"]), I(a["a"]), I(a["b"]), I(a["c"]), D(a["k"]),
+                                        D(a["t0"])))
+        self._chk(self.L.mcx_set_dihedrals(self.h, len(d["a"]), I(d["a"]), I(d["b"]), I(d["c"]), I(d["d"]),
+                                           I(d["type"]), D(d["k"]), D(d["phi0"]), D(d["mult"])))
+        self._chk(self.L.mcx_set_impropers(self.h, len(m["a"]), I(m["a"]), I(m["b"]), I(m["c"]), I(m["d"]),
+                                           I(m["type"]), D(m["k"]), D(m["phi0"]), D(m["mult"])))
+
+    def build_exclusions(self):
+        self._chk(self.L.mcx_build_exclusions(self.h))
+
+    def set_coords(self, x, y, z):
+        x, y, z = _d(x), _d(y), _d(z)
+        self._chk(self.L.mcx_set_coords(self.h, _p(x, C.c_double), _p(y, C.c_double), _p(z, C.c_double)))
+
+    def get_coords(self):
+        x, y, z = np.zeros(self.n), np.zeros(self.n), np.zeros(self.n)
+        self._chk(self.L.mcx_get_coords(self.h, _p(x, C.c_double), _p(y, C.c_double), _p(z, C.c_double)))
+        return x, y, z
+
+    # ---- lists ----
+    def update_list(self) -> int:
+        n = C.c_longlong(0)
+        self._chk(self.L.mcx_update_list(self.h, C.byref(n)))
+        return n.value
+
+    def get_list(self):
+        cnt = np.zeros(self.n, np.int32)
+        self._chk(self.L.mcx_get_list(self.h, _p(cnt, C.c_int), None, 0))
+        flat = np.zeros(max(int(cnt.sum()), 1), np.int32)
+        self._chk(self.L.mcx_get_list(self.h, _p(cnt, C.c_int), _p(flat, C.c_int), flat.size))
+        return cnt, flat[: int(cnt.sum())]
+
+    def get_exclusions(self):
+        ep = np.zeros(self.n, np.int32)
+        nidx = C.c_longlong(0)
+        n14 = C.c_int(0)
+        self._chk(self.L.mcx_get_exclusions(self.h, _p(ep, C.c_int), None, 0, C.byref(nidx), None, 0, C.byref(n14)))
+        idx = np.zeros(max(nidx.value, 1), np.int32)
+        p14 = np.zeros(max(2 * n14.value, 1), np.int32)
+        self._chk(self.L.mcx_get_exclusions(self.h, _p(ep, C.c_int), _p(idx, C.c_int), idx.size, C.byref(nidx),
+                                            _p(p14, C.c_int), p14.size, C.byref(n14)))
+        return ep, idx[: nidx.value], p14[: 2 * n14.value].reshape(-1, 2)
+
+    # ---- energies ----
+    def energy(self):
+        e = np.zeros(10)
+        self._chk(self.L.mcx_energy(self.h, _p(e, C.c_double)))
+        return e
+
+    def delta_energy(self, first, last, nx, ny, nz):
+        nx, ny, nz = _d(nx), _d(ny), _d(nz)
+        out = C.c_double(0)
+        self._chk(self.L.mcx_delta_energy(self.h, int(first), int(last), _p(nx, C.c_double), _p(ny, C.c_double),
+                                          _p(nz, C.c_double), C.byref(out)))
+        return out.value
+
+    def grad(self):
+        out = [np.zeros(self.n) for _ in range(6)]
+        self._chk(self.L.mcx_nonbonded_grad(self.h, *[_p(a, C.c_double) for a in out]))
+        return out
+
+    def time_kernel(self, which, reps=20):
+        ms = C.c_double(0)
+        self._chk(self.L.mcx_time_kernel(self.h, int(which), int(reps), C.byref(ms)))
+        return ms.value
+
+    def mc_run(self, moves: Sequence[S.MoveType], nsteps, seed, rng_mode=MC_REPLAY, list_every=50, record=True):
+        nt = len(moves)
+        arr = (MoveTypeC * nt)()
+        keep = []
+        for k, m in enumerate(moves):
+            arr[k].type, arr[k].mode, arr[k].dmax, arr[k].target, arr[k].each = m.type, m.mode, m.dmax, m.target, m.each
+            if m.sele is not None:
+                sa = _i(m.sele); keep.append(sa); arr[k].sele = _p(sa, C.c_int)
+        st = MCStatsC()
+        acc = np.zeros(max(nsteps, 1), np.uint8) if record else None
+        de = np.zeros(max(nsteps, 1)) if record else None
+        ty = np.zeros(max(nsteps, 1), np.int32) if record else None
+        ins = np.zeros(max(nsteps, 1), np.int32) if record else None
+        self._chk(self.L.mcx_mc_run(self.h, arr, nt, int(nsteps), C.c_uint64(seed), int(rng_mode), int(list_every),
+                                    C.byref(st), _p(acc, C.c_ubyte), _p(de, C.c_double), _p(ty, C.c_int),
+                                    _p(ins, C.c_int)))
+        out = dict(etot=st.etot, trials=np.array(st.trials[:nt]), accepted=np.array(st.accepted[:nt]),
+                   dmax=np.array(st.dmax[:nt]), list_updates=st.list_updates, seconds=st.seconds)
+        if record:
+            out.update(acc=acc[:nsteps], de=de[:nsteps], mvtyp=ty[:nsteps], mvinst=ins[:nsteps])
+        return out
+
+
+def from_system(s: S.System, device: int = 0) -> Handle:
+    """Creates a handle holding System `s` (the Parser_XML + IO_MDBAS +
+    List_nonBonded set-up sequence, src/Parser.cpp:77-215)."""
+    h = Handle(s.natom, device)
+    h.set_box(s.pbtype, *s.box)
+    h.set_cut(s.cutmode, s.ctoff, s.cton, s.dcut, s.temp)
+    h.set_atoms(s.charge, s.eps, s.sig, s.eps14, s.sig14, s.mass, s.frozen, s.resid)
+    h.set_terms(s)
+    h.build_exclusions()
+    h.set_coords(s.x, s.y, s.z)
+    return h
+
+
+class FField:
+    """Mirror of the reference's FField call surface (include/FField.hpp:52-153)
+    backed by the gfx950 kernels."""
+
+    def __init__(self, s: S.System, device: int = 0):
+        self.sys = s
+        self.h = from_system(s, device)
+        if s.cutmode != S.CUT_FULL:
+            self.h.update_list()      # List_nonBonded ctor builds the list (src/List_nonBonded.cpp:62-100)
+
+    def getE(self) -> float:
+        return float(self.h.energy()[0])
+
+    def getE10(self) -> np.ndarray:
+        """getE(double ener[10]): {tot, pot, kin, elec, vdw, bond, ang, ub, dihe, impr}."""
+        return self.h.energy()
+
+    def askListUpdate(self, st: int = 0):
+        """FField::askListUpdate (src/FField.cpp:222-235): no-op in FULL mode."""
+        if self.sys.cutmode != S.CUT_FULL:
+            self.h.update_list()
+
+    def getCutMode(self):
+        return self.sys.cutmode
+
+    # List_nonBonded accessors (src/List_nonBonded.cpp:1343-1376)
+    def getNeighPair(self):
+        return self.h.get_list()[0]
+
+    def getNeighList(self):
+        cnt, flat = self.h.get_list()
+        off = np.concatenate([[0], np.cumsum(cnt)])
+        return [flat[off[i]:off[i + 1]] for i in range(self.sys.natom)]
+
+    def getNeighOrder(self):
+        return np.arange(self.sys.natom, dtype=np.int32)
+
+    def getExclPair(self):
+        return self.h.get_exclusions()[0]
+
+    def getExclList(self):
+        ep, idx, _ = self.h.get_exclusions()
+        off = np.concatenate([[0], np.cumsum(ep)])
+        return [idx[off[i]:off[i + 1]] for i in range(self.sys.natom)]
+
+    def getNeighList14(self):
+        return self.h.get_exclusions()[2].reshape(-1)
+
+    def getNPair14(self):
+        return len(self.h.get_exclusions()[2])
+
+    def set_coords(self, x, y, z):
+        self.h.set_coords(x, y, z)
+
+    def get_coords(self):
+        return self.h.get_coords()
+
+
+class MC_metropolis:
+    """MC_metropolis (src/MC_metropolis.cpp:29-204) driving FField on the device."""
+
+    def __init__(self, ff: FField, moves: Optional[Sequence[S.MoveType]] = None, nsteps: int = 0, seed: int = 1,
+                 rng_mode: int = MC_REPLAY, list_every: int = 50):
+        self.ff, self.moves = ff, list(moves if moves is not None else ff.sys.moves)
+        self.nsteps, self.seed, self.rng_mode, self.list_every = nsteps, seed, rng_mode, list_every
+
+    def run(self, record=True):
+        if self.nsteps == 0:
+            return None       # "Dummy MC simulation because nsteps = 0" (src/MC.cpp:177-181)
+        return self.ff.h.mc_run(self.moves, self.nsteps, self.seed, self.rng_mode, self.list_every, record)

@@ -451,28 +451,37 @@ def mixed_system(seed: int = 3, nside: int = 32, L: float = 99.4, nbead: int = 2
     whose oxygen lies within `clash` A of a bead are removed.  Chain charges
     U(-0.3,0.3) shifted to neutral; sigma U[3,4], eps U[0.05,0.2]."""
     ww, wfp = water_box(seed=seed, nside=nside, L=L, tag=31)
-    # chain by random walk with fixed bond length and angle near theta0
-    u = philox.uniform(seed, 32, 3 * nbead)
+    # chain by a seeded self-avoiding random walk: fixed bond length and
+    # angle, torsion drawn uniformly, retried (next draws) while the new bead
+    # comes closer than 3.5 A to any earlier non-bonded bead
+    u = philox.uniform(seed, 32, 64 * nbead)
     pos = np.zeros((nbead, 3))
     th0 = np.radians(111.0)
     pos[1] = pos[0] + np.array([1.53, 0, 0])
+    cur = 0
     for i in range(2, nbead):
         b = pos[i - 1] - pos[i - 2]
         b /= np.linalg.norm(b)
         tmp = np.array([1.0, 0, 0]) if abs(b[0]) < 0.9 else np.array([0, 1.0, 0])
         p1 = np.cross(b, tmp); p1 /= np.linalg.norm(p1)
         p2 = np.cross(b, p1)
-        phi = 2 * np.pi * u[3 * i]
-        d = -np.cos(th0) * b + np.sin(th0) * (np.cos(phi) * p1 + np.sin(phi) * p2)
-        pos[i] = pos[i - 1] + 1.53 * d
+        for attempt in range(48):
+            phi = 2 * np.pi * u[cur % len(u)]
+            cur += 1
+            d = -np.cos(th0) * b + np.sin(th0) * (np.cos(phi) * p1 + np.sin(phi) * p2)
+            cand = pos[i - 1] + 1.53 * d
+            if i < 3 or np.min(np.sum((pos[:i - 2] - cand) ** 2, 1)) > 3.5 ** 2:
+                break
+        pos[i] = cand
     pos -= pos.mean(0)
-    # remove clashing waters (min-image distance of O to any bead)
-    O = ww.coords[0::3]
-    keep = np.ones(len(O), dtype=bool)
+    # remove waters with any atom closer than `clash` to a bead (min image)
+    W = ww.coords.reshape(-1, 3, 3)
+    keep = np.ones(len(W), dtype=bool)
     for start in range(0, nbead, 256):
-        d = O[:, None, :] - pos[None, start:start + 256, :]
-        d -= L * np.round(d / L)
-        keep &= ~(np.min(np.einsum("ijk,ijk->ij", d, d), 1) < clash * clash)
+        for a in range(3):
+            d = W[:, a, None, :] - pos[None, start:start + 256, :]
+            d -= L * np.round(d / L)
+            keep &= ~(np.min(np.einsum("ijk,ijk->ij", d, d), 1) < clash * clash)
     nw = int(keep.sum())
     widx = np.repeat(np.nonzero(keep)[0] * 3, 3) + np.tile([0, 1, 2], nw)
     n = 3 * nw + nbead

@@ -753,7 +753,10 @@ static void centre_of_mass(const orc_system *s, int a, int b, double c[3])
  * An instance is one atom range [first,last] plus a pivot (-1 = COM). */
 typedef struct orc_inst { int first, last, pivot; } orc_inst;
 
-static int residue_range(const orc_system *s, int atomidx, int *a1, int *a2)
+/* gtrsfl :630-697.  resLast[k] = last atom of residue k (resLast[0] = -1);
+ * the bounds test is inclusive on both sides, so for the last atom of a
+ * residue the search can return the following residue. */
+static int *residue_last_table(const orc_system *s, int *nres)
 {
     const int N = s->natom;
     int iresl = 0;
@@ -763,23 +766,29 @@ static int residue_range(const orc_system *s, int atomidx, int *a1, int *a2)
     int last = 1, idx = 0;
     for (int it = 0; it < N; it++) if (s->resid[it] > last) { last++; resLast[++idx] = it - 1; }
     resLast[iresl] = N - 1;
-    int iresf = 1, ok = 0;
+    *nres = iresl;
+    return resLast;
+}
+
+static int residue_range(const int *resLast, int nres, int atomidx, int *a1, int *a2)
+{
+    int iresf = 1, iresl = nres;
     while (iresf <= iresl) {
         int mid = (iresl + iresf) / 2;
         if (resLast[mid - 1] <= atomidx && resLast[mid] >= atomidx) {
-            *a1 = resLast[mid - 1] + 1; *a2 = resLast[mid]; ok = 1; break;
+            *a1 = resLast[mid - 1] + 1; *a2 = resLast[mid]; return 1;
         } else if (resLast[mid] <= atomidx) iresf = mid + 1;
         else iresl = mid - 1;
     }
-    free(resLast);
-    return ok;
+    return 0;
 }
 
 static orc_inst *build_instances(const orc_system *s, const orc_movetype *t, int *n)
 {
     const int N = s->natom;
     orc_inst *v = (orc_inst *)malloc(sizeof(orc_inst) * (N + 1));
-    int c = 0;
+    int c = 0, nres = 0;
+    int *resLast = residue_last_table(s, &nres);
     if (t->mode == ORC_MODE_ALL) {
         /* makeMoveList gives the selected ranges; only a contiguous "all"
          * selection is supported here */
@@ -789,7 +798,7 @@ static orc_inst *build_instances(const orc_system *s, const orc_movetype *t, int
             if (t->sele && !t->sele[i]) continue;
             if (t->mode == ORC_MODE_RESIDUE) {
                 int a1 = i, a2 = i;
-                residue_range(s, i, &a1, &a2);
+                residue_range(resLast, nres, i, &a1, &a2);
                 v[c].first = a1; v[c].last = a2;
             } else {
                 v[c].first = i; v[c].last = i;
@@ -798,6 +807,7 @@ static orc_inst *build_instances(const orc_system *s, const orc_movetype *t, int
             c++;
         }
     }
+    free(resLast);
     *n = c;
     return v;
 }
@@ -902,7 +912,7 @@ void orc_nonbonded_switch_grad(orc_system *s, double *gx, double *gy, double *gz
             if (r2 > ctoff2) continue;
             double r = sqrt(r2), rinv = 1.0 / r;
             double sij = s->sig[i] + s->sig[j];
-            double u = sij * rinv, u6 = X6(u), u12 = u6 * u6;
+            double u = sij * rinv, u6 = X6(u), u12 = u6 * u6;   /* u = 0 when sij = 0 */
             double eij = s->eps[i] * s->eps[j];
             double qq = s->charge[i] * s->charge[j];
             double Ee = C * qq * rinv;
@@ -923,8 +933,10 @@ void orc_nonbonded_switch_grad(orc_system *s, double *gx, double *gy, double *gz
             /* parameters */
             double dv_deij = S * 4.0 * (u12 - u6);
             geps[i] += dv_deij * s->eps[j]; geps[j] += dv_deij * s->eps[i];
-            double dv_dsij = S * 4.0 * eij * (12.0 * u12 - 6.0 * u6) / sij;
-            gsig[i] += dv_dsij; gsig[j] += dv_dsij;
+            if (sij != 0.0) {
+                double dv_dsij = S * 4.0 * eij * (12.0 * u12 - 6.0 * u6) / sij;
+                gsig[i] += dv_dsij; gsig[j] += dv_dsij;
+            }
             double de_dqq = S * C * rinv;
             gq[i] += de_dqq * s->charge[j]; gq[j] += de_dqq * s->charge[i];
         }

@@ -1,0 +1,217 @@
+"""Parity of the HIP path (through the C ABI) against the FP64 oracle.
+
+Tolerances (stated per quantity):
+  * total energy: relative 1e-6 (FP32 pair arithmetic, FP64 sums);
+  * elec / vdw components: 1e-6 of max(|component|, |total|);
+  * bonded components (FP64 on the device): relative 1e-9;
+  * pair list: identical sets, rows in the same ascending order (bit-exact);
+  * per-move dE: |dE_gpu - dE_fp64| <= 1e-5 * max(|dE|, kT) with
+    kT = 0.83144621 * T iu (0.596 kcal/mol at 300 K);
+  * replayed Metropolis runs: identical accept decisions except a move
+    whose dE lies within the dE tolerance of the acceptance threshold
+    (after such a move the two chains legitimately diverge).
+"""
+import numpy as np
+import pytest
+
+from mc_cpp_amd import mcx, systems
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(s):
+    ref = s.copy()
+    o = Oracle(ref)
+    return o, ref
+
+
+def _check_energy(eg, eo, tag):
+    tot = abs(eo[0])
+    assert abs(eg[0] - eo[0]) <= 1e-6 * tot, (tag, "tot", eg[0], eo[0])
+    for k, name in ((3, "elec"), (4, "vdw")):
+        assert abs(eg[k] - eo[k]) <= 1e-6 * max(abs(eo[k]), tot), (tag, name, eg[k], eo[k])
+    for k, name in ((5, "bond"), (6, "ang"), (7, "ub"), (8, "dihe"), (9, "impr")):
+        assert abs(eg[k] - eo[k]) <= 1e-9 * max(abs(eo[k]), 1e-6), (tag, name, eg[k], eo[k])
+
+
+def _check_list(h, o):
+    cnt_o, off_o, flat_o = o.list_csr()
+    cnt_g, flat_g = h.get_list()
+    assert np.array_equal(cnt_o, cnt_g)
+    assert np.array_equal(flat_o, flat_g)
+    return int(cnt_g.sum())
+
+
+SYSTEMS = {
+    "argon864": lambda: systems.argon_fcc(seed=1)[0],
+    "water1536": lambda: systems.water_box(seed=3, nside=8, L=24.85)[0],
+    "mixed_small": lambda: systems.mixed_system(seed=3, nside=8, L=24.85, nbead=60)[0],
+}
+
+
+@pytest.mark.parametrize("name", list(SYSTEMS))
+def test_switch_energy_and_list(name):
+    s = SYSTEMS[name]()
+    o, ref = _pair(s)
+    o.update_list()
+    eo = o.getE()
+    ff = mcx.FField(s)
+    npairs = _check_list(ff.h, o)
+    assert npairs > 0
+    _check_energy(ff.getE10(), eo, name)
+
+
+@pytest.mark.parametrize("name", list(SYSTEMS))
+def test_full_energy(name):
+    s = SYSTEMS[name]()
+    s.cutmode = systems.CUT_FULL
+    o, ref = _pair(s)
+    eo = o.getE()
+    ff = mcx.FField(s)
+    _check_energy(ff.getE10(), eo, name + "/full")
+
+
+def test_water_box_12288_energy_and_list():
+    """Config-2 box at full size: 12,288 atoms, 7.07 M list pairs."""
+    s, _ = systems.water_box(seed=2)
+    o, ref = _pair(s)
+    o.update_list()
+    eo = o.getE()
+    ff = mcx.FField(s)
+    n = _check_list(ff.h, o)
+    assert 6.5e6 < n < 7.5e6
+    _check_energy(ff.getE10(), eo, "water12288")
+    s2 = s.copy()
+    s2.cutmode = systems.CUT_FULL
+    o2 = Oracle(s2)
+    ff2 = mcx.FField(s2)
+    _check_energy(ff2.getE10(), o2.getE(), "water12288/full")
+
+
+def test_list_after_coordinate_update_and_boundaries():
+    """Atoms placed exactly at +-L/2 and pairs at the list radius."""
+    s, _ = systems.argon_fcc(seed=9, ncell=3, L=17.2)
+    L = s.box[0]
+    s.x[0], s.y[0], s.z[0] = L / 2, -L / 2, 0.0
+    s.x[1], s.y[1], s.z[1] = -L / 2 + 14.0, -L / 2, 0.0   # exactly at ctoff + dcut from atom 0 through the image
+    o, ref = _pair(s)
+    o.update_list()
+    ff = mcx.FField(s)
+    _check_list(ff.h, o)
+    # move every atom a little and rebuild through askListUpdate
+    rng = np.random.default_rng(3)
+    for arr in (s.x, s.y, s.z):
+        arr += rng.uniform(-0.4, 0.4, s.natom)
+    ref.x[:], ref.y[:], ref.z[:] = s.x, s.y, s.z
+    ff.set_coords(s.x, s.y, s.z)
+    ff.askListUpdate(50)
+    o.update_list()
+    _check_list(ff.h, o)
+
+
+def test_frozen_pairs_excluded():
+    s, _ = systems.argon_fcc(seed=4)
+    s.frozen[: s.natom // 2] = 1
+    o, ref = _pair(s)
+    o.update_list()
+    ff = mcx.FField(s)
+    _check_list(ff.h, o)
+
+
+def _kT(s):
+    return 0.83144621 * s.temp
+
+
+@pytest.mark.parametrize("name", ["argon864", "water1536", "mixed_small"])
+def test_delta_energy(name):
+    s = SYSTEMS[name]()
+    o, ref = _pair(s)
+    o.update_list()
+    ff = mcx.FField(s)
+    rng = np.random.default_rng(11)
+    x, y, z = ref.x, ref.y, ref.z
+    for trial in range(30):
+        i = int(rng.integers(0, s.natom))
+        r = s.resid[i]
+        idx = np.nonzero(s.resid == r)[0]
+        a, b = int(idx[0]), int(idx[-1])
+        d = rng.uniform(-0.3, 0.3, 3)
+        nx, ny, nz = x[a:b + 1] + d[0], y[a:b + 1] + d[1], z[a:b + 1] + d[2]
+        de_g = ff.h.delta_energy(a, b, nx, ny, nz)
+        e_old = o.moving_set_energy(a, b)
+        sx, sy, sz = x[a:b + 1].copy(), y[a:b + 1].copy(), z[a:b + 1].copy()
+        x[a:b + 1], y[a:b + 1], z[a:b + 1] = nx, ny, nz
+        e_new = o.moving_set_energy(a, b)
+        x[a:b + 1], y[a:b + 1], z[a:b + 1] = sx, sy, sz
+        de_o = e_new - e_old
+        tol = 1e-5 * max(abs(de_o), _kT(s))
+        assert abs(de_g - de_o) <= tol, (name, trial, a, b, de_g, de_o)
+
+
+def _compare_replay(res, ro, beta, tol_fn):
+    acc_g, acc_o = res["acc"], ro["acc"]
+    head = [(int(res["mvtyp"][k]), int(ro["mvtyp"][k]), int(res["mvinst"][k]), int(ro["mvinst"][k]),
+             float(res["de"][k]), float(ro["de"][k])) for k in range(min(5, len(acc_g)))]
+    print("first steps (typ gpu/fp64, inst gpu/fp64, dE gpu/fp64):", head)
+    mism = np.nonzero(acc_g != acc_o)[0]
+    n_ok = len(acc_g) if len(mism) == 0 else int(mism[0])
+    # dE agreement on the common prefix
+    for k in range(n_ok):
+        assert abs(res["de"][k] - ro["de"][k]) <= tol_fn(ro["de"][k]), (k, res["de"][k], ro["de"][k])
+    if len(mism):
+        k = int(mism[0])
+        # a legitimate divergence: the two dE straddle the acceptance threshold
+        assert abs(res["de"][k] - ro["de"][k]) <= tol_fn(ro["de"][k]), (k, res["de"][k], ro["de"][k])
+    return n_ok
+
+
+def test_mc_replay_argon_1e5():
+    """1e5 replayed single-atom moves on the config-1 argon box."""
+    s, _ = systems.argon_fcc(seed=1)
+    o, ref = _pair(s)
+    o.update_list()
+    ff = mcx.FField(s)
+    n = 100000
+    res = mcx.MC_metropolis(ff, nsteps=n, seed=20240601).run()
+    ro = o.mc_run(n, 20240601, mode=1)
+    kT = _kT(s)
+    n_ok = _compare_replay(res, ro, None, lambda de: 1e-5 * max(abs(de), kT))
+    assert n_ok == n, "accept sequence diverged at step %d" % n_ok
+    assert np.array_equal(res["trials"], np.bincount(ro["mvtyp"], minlength=1))
+    x, y, z = ff.get_coords()
+    assert np.max(np.abs(x - ref.x)) < 1e-9
+
+
+@pytest.mark.parametrize("which", ["trn", "rot", "both"])
+def test_mc_replay_water_residue_moves(which):
+    s, _ = systems.water_box(seed=3, nside=8, L=24.85)
+    if which == "trn":
+        s.moves = s.moves[:1]
+    elif which == "rot":
+        s.moves = s.moves[1:]
+    o, ref = _pair(s)
+    o.update_list()
+    ff = mcx.FField(s)
+    n = 3000
+    res = mcx.MC_metropolis(ff, nsteps=n, seed=777).run()
+    ro = o.mc_run(n, 777, mode=1)
+    kT = _kT(s)
+    n_ok = _compare_replay(res, ro, None, lambda de: 1e-5 * max(abs(de), kT))
+    assert n_ok == n
+    # running total matches a fresh evaluation of the final state
+    assert abs(res["etot"] - ff.getE()) <= 1e-6 * abs(ff.getE())
+
+
+def test_gradient_vs_fp64_and_fd():
+    s, _ = systems.water_box(seed=3, nside=8, L=24.85)
+    o, ref = _pair(s)
+    o.update_list()
+    ff = mcx.FField(s)
+    gx, gy, gz, ge, gs, gq = ff.h.grad()
+    ox, oy, oz, oe, os_, oq = o.grad_switch()
+    scale = np.max(np.abs(ox))
+    for g_, o_ in ((gx, ox), (gy, oy), (gz, oz)):
+        assert np.max(np.abs(g_ - o_)) <= 1e-4 * scale
+    for g_, o_ in ((ge, oe), (gs, os_), (gq, oq)):
+        assert np.max(np.abs(g_ - o_)) <= 1e-4 * max(np.max(np.abs(o_)), 1e-12)
