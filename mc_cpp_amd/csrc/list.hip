@@ -86,72 +86,114 @@ __global__ void k_cell_fill(DevSys s, CellGrid g)
     const int c = g.cell_of[a];
     const int p = g.cell_start[c] + atomicAdd(&g.cursor[c], 1);
     g.cell_atoms[p] = a;
+    g.cpos[p] = make_double4(s.x[a], s.y[a], s.z[a], (double)(2 * a + (s.frozen[a] ? 1 : 0)));
 }
 
+// Tools::distance2 + the r2 <= cutnb2 test, operation by operation
 template <bool PBC>
-__device__ __forceinline__ bool pair_in_list(const DevSys &s, int i, int j, double xi, double yi, double zi)
+__device__ __forceinline__ bool within(const DevSys &s, double4 pj, double xi, double yi, double zi)
 {
-    double dx = s.x[j] - xi;
-    double dy = s.y[j] - yi;
-    double dz = s.z[j] - zi;
+    double dx = pj.x - xi;
+    double dy = pj.y - yi;
+    double dz = pj.z - zi;
     if (PBC) {
         dx -= s.L[0] * ref_rint(s.rL[0] * dx);
         dy -= s.L[1] * ref_rint(s.rL[1] * dy);
         dz -= s.L[2] * ref_rint(s.rL[2] * dz);
     }
     const double r2 = dx * dx + dy * dy + dz * dz;
-    if (!(r2 <= s.cutnb2)) return false;
-    if (s.frozen[i] && s.frozen[j]) return false;
-    const int lo = i < j ? i : j, hi = i < j ? j : i;
-    for (int k = s.excl_off[lo]; k < s.excl_off[lo + 1]; k++)
-        if (s.excl_idx[k] == hi) return false;
-    return true;
+    return r2 <= s.cutnb2;
 }
+
+constexpr int kMaxRuns = 64;   // 2 runs for each of up to 25 (y, z) cell columns, rounded up
+constexpr int kUnroll = 4;
 
 template <bool PBC>
 __global__ __launch_bounds__(256) void k_build_rows(DevSys s, CellGrid g, int words, int wpb)
 {
     extern __shared__ uint32_t bitmap[];
+    __shared__ int run_start[4][kMaxRuns], run_len[4][kMaxRuns];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const int i = blockIdx.x * wpb + w;
+    const bool active = i < s.n;
     uint32_t *B = bitmap + (size_t)w * words;
     for (int k = lane; k < words; k += 64) B[k] = 0u;
-    __syncthreads();
-    if (i < s.n) {
 
-    const double xi = s.x[i], yi = s.y[i], zi = s.z[i];
-    const int ci = g.cell_of[i];
-    const int cx = ci % g.nc[0], cy = (ci / g.nc[0]) % g.nc[1], cz = ci / (g.nc[0] * g.nc[1]);
-    // neighbour cell ranges; a dimension with fewer than 2*reach+1 cells is
-    // searched whole so that no cell is visited twice
-    int lo[3], hi[3];
-    const int cc[3] = {cx, cy, cz};
-    for (int d = 0; d < 3; d++) {
-        if (2 * g.reach[d] + 1 >= g.nc[d]) { lo[d] = 0; hi[d] = g.nc[d] - 1; }
-        else { lo[d] = cc[d] - g.reach[d]; hi[d] = cc[d] + g.reach[d]; }
+    // phase 1: the candidate runs of row i.  One lane per (y, z) column of
+    // neighbour cells; the x cells of a column are contiguous in cell
+    // order, except where the range wraps (then two runs).
+    double xi = 0.0, yi = 0.0, zi = 0.0;
+    bool fi = false;
+    int ncol = 0;
+    if (active) {
+        xi = s.x[i]; yi = s.y[i]; zi = s.z[i];
+        fi = s.frozen[i] != 0;
+        const int ci = g.cell_of[i];
+        const int cc[3] = {ci % g.nc[0], (ci / g.nc[0]) % g.nc[1], ci / (g.nc[0] * g.nc[1])};
+        int lo[3], cnt[3];
+        for (int d = 0; d < 3; d++) {
+            if (2 * g.reach[d] + 1 >= g.nc[d]) { lo[d] = 0; cnt[d] = g.nc[d]; }
+            else { lo[d] = cc[d] - g.reach[d]; cnt[d] = 2 * g.reach[d] + 1; }
+        }
+        ncol = cnt[1] * cnt[2];
+        for (int col = lane; col < kMaxRuns / 2; col += 64) {
+            int s0 = 0, l0 = 0, s1 = 0, l1 = 0;
+            if (col < ncol) {
+                const int wy = (lo[1] + col % cnt[1] + g.nc[1]) % g.nc[1];
+                const int wz = (lo[2] + col / cnt[1] + g.nc[2]) % g.nc[2];
+                const int base = (wz * g.nc[1] + wy) * g.nc[0];
+                int x0 = lo[0], x1 = lo[0] + cnt[0] - 1, a0 = 0, b0 = -1;
+                if (x0 < 0) { a0 = x0 + g.nc[0]; b0 = g.nc[0] - 1; x0 = 0; }
+                else if (x1 >= g.nc[0]) { a0 = 0; b0 = x1 - g.nc[0]; x1 = g.nc[0] - 1; }
+                s0 = g.cell_start[base + x0];
+                l0 = g.cell_start[base + x1] + g.cell_count[base + x1] - s0;
+                if (b0 >= a0) {
+                    s1 = g.cell_start[base + a0];
+                    l1 = g.cell_start[base + b0] + g.cell_count[base + b0] - s1;
+                }
+            }
+            run_start[w][2 * col] = s0; run_len[w][2 * col] = l0;
+            run_start[w][2 * col + 1] = s1; run_len[w][2 * col + 1] = l1;
+        }
     }
-    for (int zc = lo[2]; zc <= hi[2]; zc++) {
-        const int wz = (zc + g.nc[2]) % g.nc[2];
-        for (int yc = lo[1]; yc <= hi[1]; yc++) {
-            const int wy = (yc + g.nc[1]) % g.nc[1];
-            for (int xc = lo[0]; xc <= hi[0]; xc++) {
-                const int wx = (xc + g.nc[0]) % g.nc[0];
-                const int c = (wz * g.nc[1] + wy) * g.nc[0] + wx;
-                const int a0 = g.cell_start[c], a1 = a0 + g.cell_count[c];
-                for (int a = a0 + lane; a < a1; a += 64) {
-                    const int j = g.cell_atoms[a];
-                    if (j != i && pair_in_list<PBC>(s, i, j, xi, yi, zi))
+    __syncthreads();
+
+    // phase 2: stream the runs, kUnroll loads in flight per lane; every hit
+    // sets bit j of the row bitmap
+    if (active) {
+        for (int r = 0; r < 2 * ncol; r++) {
+            const int rs = run_start[w][r], rl = run_len[w][r];
+            for (int off = 0; off < rl; off += 64 * kUnroll) {
+                double4 pj[kUnroll];
+#pragma unroll
+                for (int u = 0; u < kUnroll; u++) {
+                    const int t = off + u * 64 + lane;
+                    pj[u] = t < rl ? g.cpos[rs + t] : make_double4(0.0, 0.0, 0.0, -1.0);
+                }
+#pragma unroll
+                for (int u = 0; u < kUnroll; u++) {
+                    const int code = (int)pj[u].w;
+                    const int j = code >> 1;
+                    if (code >= 0 && j != i && !(fi && (code & 1)) && within<PBC>(s, pj[u], xi, yi, zi))
                         atomicOr(&B[j >> 5], 1u << (j & 31));
                 }
             }
         }
     }
+    __syncthreads();
+    // phase 3: exclusions only remove pairs -- clear the bits of i's
+    // excluded partners
+    if (active) {
+        for (int k = s.xfull_off[i] + lane; k < s.xfull_off[i + 1]; k += 64) {
+            const int e = s.xfull_idx[k];
+            atomicAnd(&B[e >> 5], ~(1u << (e & 31)));
+        }
     }
     __syncthreads();
-    if (i >= s.n) return;
+    if (!active) return;
 
-    // ordered compaction of the bitmap
+    // phase 4: ordered compaction of the bitmap (no barriers below)
     int *row = s.nbr + (size_t)i * s.cap;
     int total = 0, nlow = 0;
     const int iw = i >> 5;
@@ -159,7 +201,6 @@ __global__ __launch_bounds__(256) void k_build_rows(DevSys s, CellGrid g, int wo
         const int k = base + lane;
         uint32_t wd = k < words ? B[k] : 0u;
         const int pc = __popc(wd);
-        // inclusive wave prefix sum
         int incl = pc;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
@@ -168,7 +209,6 @@ __global__ __launch_bounds__(256) void k_build_rows(DevSys s, CellGrid g, int wo
         }
         const int tot = __shfl(incl, 63, 64);
         int pos = total + incl - pc;
-        // atoms below i: whole words below i's word, plus the low bits of it
         int lowc = 0;
         if (k < iw) lowc = pc;
         else if (k == iw) lowc = __popc(wd & ((1u << (i & 31)) - 1u));
@@ -209,7 +249,7 @@ void launch_list_build(const DevSys &s, const CellGrid &g, hipStream_t st)
     const int words = (s.n + 31) / 32;
     const size_t per_wave = (size_t)words * 4;
     int wpb = 4;
-    while (wpb > 1 && per_wave * wpb > 64 * 1024) wpb >>= 1;
+    while (wpb > 1 && per_wave * wpb > 60 * 1024) wpb >>= 1;
     const size_t lds = per_wave * wpb;
     const int blocks = (s.n + wpb - 1) / wpb;
     if (s.pbtype == 0)

@@ -4,12 +4,11 @@
 // difference of two full-system energies (:94,122-123).  Here one
 // workgroup evaluates only the terms that touch the moved atoms:
 //   * nonbonded pairs from the moved atoms' full list rows (SWITCH) or all
-//     atoms (FULL), with the pair arithmetic in FP32 written in difference
-//     form -- r_new^2 - r_old^2 = delta.(2 d_old + delta), and 1/r, t^3,
-//     t^6-t^3 differences factored through it -- so that a small dE is not
-//     the cancellation of two large FP32 energies; the switching polynomial
-//     and all sums are FP64;
-//   * bonded and 1-4 terms through a per-atom adjacency list, FP64.
+//     atoms (FULL): each pair's old and new energy from the exact
+//     fixed-point displacement and its change, with the distance chain and
+//     Coulomb in FP64 (so the per-pair difference is exact to ~1e-15
+//     relative), LJ in FP32, the switching polynomial and all sums in FP64;
+//   * bonded and 1-4 terms through per-move term records, FP64.
 // The move itself (src/MC_metropolis.cpp:84-119; include/Move.hpp:42-183),
 // the draws (src/MC.cpp:127-156: std::mt19937 + generate_canonical, or a
 // Philox4x32-10 stream), the test of apply_criterion (:175-204) and the
@@ -21,26 +20,22 @@
 
 namespace {
 
-constexpr int kMCT = 512;
+constexpr int kMCT = 1024;
+constexpr int kPairU = 4;          // list entries in flight per thread
+constexpr int kMaxRec = 96;        // term records of one move staged in LDS
 constexpr int kMCW = kMCT / 64;
 
-struct MT {
-    uint32_t mt[624];
-    int idx;
-};
+// ---------------------------------------------------------------------------
+// Random draws.  Wavefront 0 runs the serial part of every move with all 64
+// lanes in step; it keeps a batch of 32 draws in registers (lane l holds
+// draw l) and reads the next one with a wave-uniform shuffle.  A batch is
+// refilled in parallel: 64 MT19937 outputs (twist done by the wavefront in
+// four dependency-free phases) or 16 Philox4x32-10 blocks.
 
-__device__ uint32_t mt_next(MT &g)
+constexpr int kMTN = 624;
+
+__device__ __forceinline__ uint32_t mt_temper(uint32_t y)
 {
-    if (g.idx >= 624) {
-        for (int i = 0; i < 624; i++) {
-            const uint32_t y = (g.mt[i] & 0x80000000u) | (g.mt[(i + 1) % 624] & 0x7fffffffu);
-            uint32_t v = g.mt[(i + 397) % 624] ^ (y >> 1);
-            if (y & 1u) v ^= 0x9908b0dfu;
-            g.mt[i] = v;
-        }
-        g.idx = 0;
-    }
-    uint32_t y = g.mt[g.idx++];
     y ^= (y >> 11);
     y ^= (y << 7) & 0x9d2c5680u;
     y ^= (y << 15) & 0xefc60000u;
@@ -48,11 +43,48 @@ __device__ uint32_t mt_next(MT &g)
     return y;
 }
 
-// libstdc++ generate_canonical<double,53> over two 32-bit outputs
-__device__ double mt_canonical(MT &g)
+__device__ __forceinline__ uint32_t mt_new(uint32_t cur, uint32_t nxt, uint32_t far)
 {
-    double sum = (double)mt_next(g);
-    sum += (double)mt_next(g) * 4294967296.0;
+    const uint32_t y = (cur & 0x80000000u) | (nxt & 0x7fffffffu);
+    uint32_t v = far ^ (y >> 1);
+    if (y & 1u) v ^= 0x9908b0dfu;
+    return v;
+}
+
+// MT19937 state regeneration by one wavefront.  New mt[i] depends on old
+// mt[i+1] and on mt[i+397] (old for i < 227, new for i >= 227), so the
+// ranges [0,227), [227,454), [454,623) and {623} are each free of
+// intra-phase dependencies; every phase reads all its inputs before writing.
+__device__ void mt_twist_wave(uint32_t *mt, int lane)
+{
+    const int lo[4] = {0, 227, 454, 623};
+    const int hi[4] = {227, 454, 623, 624};
+    for (int ph = 0; ph < 4; ph++) {
+        uint32_t v[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int i = lo[ph] + r * 64 + lane;
+            if (i < hi[ph]) v[r] = mt_new(mt[i], mt[(i + 1) % kMTN], mt[(i + 397) % kMTN]);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int i = lo[ph] + r * 64 + lane;
+            if (i < hi[ph]) mt[i] = v[r];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+__device__ __forceinline__ double canonical2(uint32_t lo, uint32_t hi)
+{
+    // libstdc++ generate_canonical<double,53> over two 32-bit outputs
+    double sum = (double)lo;
+    sum += (double)hi * 4294967296.0;
     double r = sum / 18446744073709551616.0;
     if (r >= 1.0) r = 0.99999999999999989;
     return r;
@@ -70,25 +102,52 @@ __device__ uint4 philox(uint4 c, uint2 k)
     return c;
 }
 
-struct Rng {
-    MT *mt;           // replay stream (LDS) or null
-    uint2 key;        // Philox key
-    uint32_t chain;   // Philox stream id
-    uint64_t ctr;     // Philox block counter
-    uint4 buf;
-    int nbuf;
-    __device__ double u01()
+struct WaveDraws {
+    double v;          // this lane's draw of the current batch
+    int cur;           // next draw (uniform)
+    int mt_idx;        // MT19937 output index (uniform)
+    uint64_t ctr;      // Philox block counter (uniform)
+    bool replay;
+    uint2 key;
+    uint32_t chain;
+    uint32_t *mt;      // LDS state
+
+    __device__ void refill(int lane)
     {
-        if (mt) return mt_canonical(*mt);
-        if (nbuf == 0) {
-            buf = philox(make_uint4((uint32_t)ctr, (uint32_t)(ctr >> 32), chain, 0x6d63u), key);
-            ctr++;
-            nbuf = 2;
+        if (replay) {
+            const int p = mt_idx + lane;
+            uint32_t y = p < kMTN ? mt[p] : 0u;
+            if (mt_idx + 64 > kMTN) {
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                mt_twist_wave(mt, lane);
+                if (p >= kMTN) y = mt[p - kMTN];
+            }
+            // a batch ending exactly at 624 leaves the index at 624 (twist due
+            // before the next output); only a batch that crossed it wraps
+            mt_idx = mt_idx + 64 > kMTN ? mt_idx + 64 - kMTN : mt_idx + 64;
+            const uint32_t out = mt_temper(y);
+            const uint32_t lo = __shfl(out, (2 * lane) & 63, 64), hi = __shfl(out, (2 * lane + 1) & 63, 64);
+            v = canonical2(lo, hi);
+        } else {
+            uint4 w = make_uint4(0, 0, 0, 0);
+            if (lane < 16) w = philox(make_uint4((uint32_t)(ctr + lane), (uint32_t)((ctr + lane) >> 32), chain, 0x6d63u), key);
+            ctr += 16;
+            const int src = (lane >> 1) & 15;
+            const uint32_t x0 = __shfl(w.x, src, 64), x1 = __shfl(w.y, src, 64);
+            const uint32_t x2 = __shfl(w.z, src, 64), x3 = __shfl(w.w, src, 64);
+            const uint32_t a = (lane & 1) ? x2 : x0, b = (lane & 1) ? x3 : x1;
+            v = ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) / 9007199254740992.0;
         }
-        uint32_t a, b;
-        if (nbuf == 2) { a = buf.x; b = buf.y; } else { a = buf.z; b = buf.w; }
-        nbuf--;
-        return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) / 9007199254740992.0;
+        cur = 0;
+    }
+    __device__ double next(int lane)
+    {
+        if (cur == 32) refill(lane);
+        const double u = __shfl(v, cur, 64);
+        cur++;
+        return u;
     }
 };
 
@@ -101,11 +160,21 @@ __device__ __forceinline__ double wsum(double v)
 
 // Shared state of one move.
 struct MoveLDS {
-    int first, last, nm, total, acc, type, inst;
+    int first, last, nm, total, acc, type, inst, pivot;
     double r[3], R[3][3], piv[3];
     double tx[kMaxMoveAtoms], ty[kMaxMoveAtoms], tz[kMaxMoveAtoms];
     double dx[kMaxMoveAtoms], dy[kMaxMoveAtoms], dz[kMaxMoveAtoms];   // unwrapped displacement of each moved atom
     int pref[kMaxMoveAtoms + 1];
+    int4 ip[kMaxMoveAtoms];      // fixed-point mirror of the moved atoms (old positions)
+    float2 me[kMaxMoveAtoms];    // (eps, sig) of the moved atoms
+    int mc[kMaxMoveAtoms];       // list-row length (or n) of the moved atoms
+    double ox[kMaxMoveAtoms], oy[kMaxMoveAtoms], oz[kMaxMoveAtoms];   // old coordinates
+    double ms[kMaxMoveAtoms];    // masses (centre-of-mass pivots)
+    double xpiv[3];              // pivot coordinates when the pivot is not moved
+    int ra0, ra1;                // term-record range of the move (ra1 < 0: use the per-atom adjacency)
+    int4 rid[kMaxRec];
+    int rkk[kMaxRec], rty[kMaxRec];
+    double4 rpar[kMaxRec];
     double red[3][kMCW];
     double de;
 };
@@ -113,13 +182,20 @@ struct MoveLDS {
 // dE contribution of pair (i, j) with i moved (index a) and j either moved
 // (index b >= 0, new coordinates tx/ty/tz[b]) or fixed (b < 0).
 // Adds q_i-scaled elec_raw and e_i-scaled vdw_raw.
-__device__ void pair_delta(const DevSys &s, const MoveLDS &m, int i, int a, int j, int b, double &dec, double &dev)
+__device__ void pair_delta(const DevSys &s, const MoveLDS &m, int i, int a, int j, int b, const int4 pj,
+                           const float2 ej, double &dec, double &dev)
 {
-    const float qi = __int_as_float(s.ipos[i].w), qj = __int_as_float(s.ipos[j].w);
-    const float2 ei = s.es[i];
-    const float2 ej = s.es[j];
-    double o[3];
-    mcx_disp_A(s, i, j, o);                       // old displacement x_j - x_i
+    const int4 pi = m.ip[a];
+    const float qi = __int_as_float(pi.w), qj = __int_as_float(pj.w);
+    const float2 ei = m.me[a];
+    double o[3];                                  // old displacement x_j - x_i
+    if (s.pbtype != 0) {
+        o[0] = (double)mcx_wdiff(pj.x, pi.x) * s.unit[0];
+        o[1] = (double)mcx_wdiff(pj.y, pi.y) * s.unit[1];
+        o[2] = (double)mcx_wdiff(pj.z, pi.z) * s.unit[2];
+    } else {
+        o[0] = s.x[j] - s.x[i]; o[1] = s.y[j] - s.y[i]; o[2] = s.z[j] - s.z[i];
+    }
     double g[3] = {-m.dx[a], -m.dy[a], -m.dz[a]};  // change of the displacement
     if (b >= 0) { g[0] += m.dx[b]; g[1] += m.dy[b]; g[2] += m.dz[b]; }
     const double nx = o[0] + g[0], ny = o[1] + g[1], nz = o[2] + g[2];
@@ -128,59 +204,31 @@ __device__ void pair_delta(const DevSys &s, const MoveLDS &m, int i, int a, int 
     bool consistent = true;
     if (s.pbtype != 0)
         consistent = fabs(nx) < 0.5 * s.L[0] && fabs(ny) < 0.5 * s.L[1] && fabs(nz) < 0.5 * s.L[2];
-    double dc, dv;
-    if (consistent) {
-        const double r2o = fma(o[2], o[2], fma(o[1], o[1], o[0] * o[0]));
-        // r_new^2 - r_old^2 = g.(2 o + g), free of cancellation
-        const double D = fma(g[2], fma(2.0, o[2], g[2]), fma(g[1], fma(2.0, o[1], g[1]), g[0] * fma(2.0, o[0], g[0])));
-        const double r2n = r2o + D;
-        const bool ino = full || r2o <= s.ctoff2, inn = full || r2n <= s.ctoff2;
-        if (!ino && !inn) return;
-        float rio32, rin32;
-        const double rio = mcx_rsqrt(r2o, rio32), rin = mcx_rsqrt(r2n, rin32);
-        const float s2 = sij * sij;
-        const float to = s2 * rio32 * rio32, tn = s2 * rin32 * rin32;
-        const float t3o = to * to * to, t3n = tn * tn * tn;
-        const double co = (double)qj * rio, cn = (double)qj * rin;
-        const float vo = ej.x * fmaf(t3o, t3o, -t3o), vn = ej.x * fmaf(t3n, t3n, -t3n);
-        const double So = full ? 1.0 : mcx_switch(s, r2o), Sn = full ? 1.0 : mcx_switch(s, r2n);
-        if (ino && inn) {
-            // 1/r_n - 1/r_o = -D / (r_o r_n (r_o + r_n)); t^3 and t^6-t^3
-            // differences factored through t_n - t_o = -s^2 D / (r_o^2 r_n^2)
-            const double dri = -D * rio * rin / fma(r2o, rio, r2n * rin);
-            const float Df = (float)D;
-            const float dt = -s2 * Df * (rio32 * rio32) * (rin32 * rin32);
-            const float dt3 = dt * fmaf(tn, tn, fmaf(tn, to, to * to));
-            const float dL = dt3 * (t3n + t3o - 1.0f);
-            const double dS = Sn - So;
-            dc = Sn * ((double)qj * dri) + co * dS;
-            dv = Sn * (double)(ej.x * dL) + (double)vo * dS;
-        } else if (inn) {
-            dc = Sn * cn;
-            dv = Sn * (double)vn;
-        } else {
-            dc = -So * co;
-            dv = -So * (double)vo;
-        }
-    } else {
-        // image changed between the old and new displacement: evaluate both
+    // the same energy function evaluated at the old and the new
+    // displacement: FP64 distance chain and Coulomb (the per-pair difference
+    // is then exact to ~1e-15 relative), FP32 LJ, FP64 switch
+    double dn[3] = {nx, ny, nz};
+    if (!consistent) {
+        // the minimum image changed between old and new displacement
         double xjn, yjn, zjn;
         if (b >= 0) { xjn = m.tx[b]; yjn = m.ty[b]; zjn = m.tz[b]; }
         else { xjn = s.x[j]; yjn = s.y[j]; zjn = s.z[j]; }
-        double dn[3] = {xjn - m.tx[a], yjn - m.ty[a], zjn - m.tz[a]};
+        dn[0] = xjn - m.tx[a]; dn[1] = yjn - m.ty[a]; dn[2] = zjn - m.tz[a];
         mcx_mimg64(s, dn);
-        double r2o, r2n, co, cn;
-        float vo, vn;
-        mcx_pair(o[0], o[1], o[2], qj, ej.x, sij, r2o, co, vo);
-        mcx_pair(dn[0], dn[1], dn[2], qj, ej.x, sij, r2n, cn, vn);
-        double So = 1.0, Sn = 1.0;
-        if (!full) {
-            So = r2o <= s.ctoff2 ? mcx_switch(s, r2o) : 0.0;
-            Sn = r2n <= s.ctoff2 ? mcx_switch(s, r2n) : 0.0;
-        }
-        dc = Sn * cn - So * co;
-        dv = Sn * (double)vn - So * (double)vo;
     }
+    const double r2o = fma(o[2], o[2], fma(o[1], o[1], o[0] * o[0]));
+    const double r2n = fma(dn[2], dn[2], fma(dn[1], dn[1], dn[0] * dn[0]));
+    const bool ino = full || r2o <= s.ctoff2, inn = full || r2n <= s.ctoff2;
+    if (!ino && !inn) return;
+    float rio32, rin32;
+    const double rio = mcx_rsqrt(r2o, rio32), rin = mcx_rsqrt(r2n, rin32);
+    const float s2 = sij * sij;
+    const float to = s2 * rio32 * rio32, tn = s2 * rin32 * rin32;
+    const float t3o = to * to * to, t3n = tn * tn * tn;
+    const double So = ino ? (full ? 1.0 : mcx_switch(s, r2o)) : 0.0;
+    const double Sn = inn ? (full ? 1.0 : mcx_switch(s, r2n)) : 0.0;
+    const double dc = (double)qj * (Sn * rin - So * rio);
+    const double dv = (double)ej.x * (Sn * (double)fmaf(t3n, t3n, -t3n) - So * (double)fmaf(t3o, t3o, -t3o));
     dec += (double)qi * dc;
     dev += (double)ei.x * dv;
 }
@@ -193,66 +241,99 @@ __device__ __forceinline__ bool excluded(const DevSys &s, int i, int j)
     return false;
 }
 
-// dE of the move described in m (trial coordinates and displacements set).
+// dE of the move described in m (trial coordinates, displacements and the
+// cached moved-atom data set by make_trial).  Wavefront 0 evaluates the
+// bonded and 1-4 terms while the other wavefronts evaluate the nonbonded
+// pairs, each thread issuing the loads of kPairU list entries together.
 // Returns the total in m.de (valid after the final barrier).
 __device__ void move_delta(const DevSys &s, const DevTerms &t, const DevMoves &mv, MoveLDS &m)
 {
     const int tid = threadIdx.x;
-    if (tid == 0) {
-        int acc = 0;
-        for (int a = 0; a < m.nm; a++) {
-            m.pref[a] = acc;
-            acc += s.cutmode == 0 ? s.n : s.cnt[m.first + a];
-        }
-        m.pref[m.nm] = acc;
-        m.total = acc;
-    }
-    __syncthreads();
     double dec = 0.0, dev = 0.0, deb = 0.0;
-    for (int e = tid; e < m.total; e += kMCT) {
-        int a = 0;
-        while (m.pref[a + 1] <= e) a++;
-        const int k = e - m.pref[a];
-        const int i = m.first + a;
-        int j;
-        if (s.cutmode == 0) {
-            j = k;
-            if (j == i) continue;
-            if (excluded(s, i, j)) continue;
-        } else {
-            j = s.nbr[(size_t)i * s.cap + k];
+    if (tid >= 64) {
+        constexpr int NP = kMCT - 64;
+        const int pt = tid - 64;
+        const int total = m.total, nm = m.nm;
+        for (int base = 0; base < total; base += kPairU * NP) {
+            int av[kPairU], jv[kPairU];
+#pragma unroll
+            for (int u = 0; u < kPairU; u++) {
+                const int e = base + u * NP + pt;
+                av[u] = -1;
+                jv[u] = -1;
+                if (e < total) {
+                    int lo = 0, hi = nm - 1;       // last a with pref[a] <= e
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (m.pref[mid] <= e) lo = mid; else hi = mid - 1;
+                    }
+                    const int k = e - m.pref[lo];
+                    av[u] = lo;
+                    jv[u] = s.cutmode == 0 ? k : s.nbr[(size_t)(m.first + lo) * s.cap + k];
+                }
+            }
+            int4 pj[kPairU];
+            float2 ej[kPairU];
+#pragma unroll
+            for (int u = 0; u < kPairU; u++) {
+                if (jv[u] >= 0) { pj[u] = s.ipos[jv[u]]; ej[u] = s.es[jv[u]]; }
+            }
+#pragma unroll
+            for (int u = 0; u < kPairU; u++) {
+                const int a = av[u], j = jv[u];
+                if (a < 0) continue;
+                const int i = m.first + a;
+                if (s.cutmode == 0 && (j == i || excluded(s, i, j))) continue;
+                int b = -1;
+                if (j >= m.first && j <= m.last) {
+                    if (j < i) continue;     // pair of two moved atoms: counted once
+                    b = j - m.first;
+                }
+                pair_delta(s, m, i, a, j, b, pj[u], ej[u], dec, dev);
+            }
         }
-        int b = -1;
-        if (j >= m.first && j <= m.last) {
-            if (j < i) continue;     // pair of two moved atoms: counted once
-            b = j - m.first;
+    } else if (m.ra1 >= 0) {
+        // bonded and 1-4 terms of the move: self-contained records, staged in
+        // LDS by load_moved (or read from global memory past kMaxRec)
+        const CoordFetch fo{s.x, s.y, s.z, nullptr, nullptr, nullptr, 0, -1};
+        const CoordFetch fn{s.x, s.y, s.z, m.tx, m.ty, m.tz, m.first, m.last};
+        for (int e = m.ra0 + tid; e < m.ra1; e += 64) {
+            const int r = e - m.ra0;
+            const bool lds = r < kMaxRec;
+            const int4 id = lds ? m.rid[r] : mv.rec_ids[e];
+            const int kk = lds ? m.rkk[r] : mv.rec_kk[e];
+            const int ty = lds ? m.rty[r] : mv.rec_type[e];
+            const double4 par = lds ? m.rpar[r] : mv.rec_par[e];
+            double e0, c0, v0, e1, c1, v1;
+            mcx_term_energy_rec(s, kk >> 28, id, ty, par, fo, e0, c0, v0);
+            mcx_term_energy_rec(s, kk >> 28, id, ty, par, fn, e1, c1, v1);
+            deb += e1 - e0;
+            dec += c1 - c0;
+            dev += v1 - v0;
         }
-        pair_delta(s, m, i, a, j, b, dec, dev);
-    }
-    // bonded and 1-4 terms touching the moved atoms, each counted at its
-    // lowest moved atom
-    const CoordFetch fo{s.x, s.y, s.z, nullptr, nullptr, nullptr, 0, -1};
-    const CoordFetch fn{s.x, s.y, s.z, m.tx, m.ty, m.tz, m.first, m.last};
-    const int a0 = mv.adj_off[m.first], a1 = mv.adj_off[m.last + 1];
-    for (int e = a0 + tid; e < a1; e += kMCT) {
-        const int g = mv.adj_term[e];
-        int owner = m.first;
-        while (mv.adj_off[owner + 1] <= e) owner++;
-        int kind = 0;
-        while (g >= t.start[kind + 1]) kind++;
-        const int k = g - t.start[kind];
-        const int4 id = t.idx[kind][k];
-        int lowest = 0x7fffffff;
-        const int ids[4] = {id.x, id.y, id.z, id.w};
-        for (int q = 0; q < 4; q++)
-            if (ids[q] >= m.first && ids[q] <= m.last && ids[q] < lowest) lowest = ids[q];
-        if (lowest != owner) continue;
-        double e0, c0, v0, e1, c1, v1;
-        mcx_term_energy(s, t, kind, k, fo, e0, c0, v0);
-        mcx_term_energy(s, t, kind, k, fn, e1, c1, v1);
-        deb += e1 - e0;
-        dec += c1 - c0;
-        dev += v1 - v0;
+    } else {
+        // bonded and 1-4 terms touching the moved atoms through the per-atom
+        // adjacency, each counted at its lowest moved atom
+        const CoordFetch fo{s.x, s.y, s.z, nullptr, nullptr, nullptr, 0, -1};
+        const CoordFetch fn{s.x, s.y, s.z, m.tx, m.ty, m.tz, m.first, m.last};
+        const int a0 = mv.adj_off[m.first], a1 = mv.adj_off[m.last + 1];
+        for (int e = a0 + tid; e < a1; e += 64) {
+            const int4 id = mv.adj_ids[e];
+            const int kk = mv.adj_kk[e];
+            int owner = m.first;
+            while (mv.adj_off[owner + 1] <= e) owner++;
+            int lowest = 0x7fffffff;
+            const int ids[4] = {id.x, id.y, id.z, id.w};
+            for (int q = 0; q < 4; q++)
+                if (ids[q] >= m.first && ids[q] <= m.last && ids[q] < lowest) lowest = ids[q];
+            if (lowest != owner) continue;
+            double e0, c0, v0, e1, c1, v1;
+            mcx_term_energy_rec(s, kk >> 28, id, mv.adj_type[e], mv.adj_par[e], fo, e0, c0, v0);
+            mcx_term_energy_rec(s, kk >> 28, id, mv.adj_type[e], mv.adj_par[e], fn, e1, c1, v1);
+            deb += e1 - e0;
+            dec += c1 - c0;
+            dev += v1 - v0;
+        }
     }
     dec = wsum(dec);
     dev = wsum(dev);
@@ -294,24 +375,74 @@ __device__ __forceinline__ double wrap1(const DevSys &s, double v, int k)
     return s.pbtype != 0 ? v - s.L[k] * mcx_ref_rint(s.rL[k] * v) : v;
 }
 
-// Trial coordinates of the moved atoms (threads 0..nm-1).
-__device__ void make_trial(const DevSys &s, MoveLDS &m)
+// Phase B: every datum of the moved atoms, the pivot and the move's term
+// records in one round of parallel loads.
+__device__ void load_moved(const DevSys &s, const DevMoves &mv, MoveLDS &m)
 {
     const int a = threadIdx.x;
     if (a < m.nm) {
         const int i = m.first + a;
-        const double x = s.x[i], y = s.y[i], z = s.z[i];
+        m.ox[a] = s.x[i]; m.oy[a] = s.y[i]; m.oz[a] = s.z[i];
+        m.ip[a] = s.ipos[i];
+        m.me[a] = s.es[i];
+        m.mc[a] = s.cutmode == 0 ? s.n : s.cnt[i];
+        m.ms[a] = mv.mass ? mv.mass[i] : 1.0;
+    } else if (a == kMaxMoveAtoms) {
+        const int pv = m.pivot;
+        if (pv >= 0 && (pv < m.first || pv > m.last)) {
+            m.xpiv[0] = s.x[pv]; m.xpiv[1] = s.y[pv]; m.xpiv[2] = s.z[pv];
+        }
+    }
+    const int r = a - 128;
+    if (r >= 0 && r < kMaxRec && m.ra0 + r < m.ra1) {
+        const int e = m.ra0 + r;
+        m.rid[r] = mv.rec_ids[e];
+        m.rkk[r] = mv.rec_kk[e];
+        m.rty[r] = mv.rec_type[e];
+        m.rpar[r] = mv.rec_par[e];
+    }
+    __syncthreads();
+}
+
+// Phase C: trial coordinates (MOVE_TRN::tr_a_b / MOVE_ROT::apply_rotation,
+// include/Move.hpp:42-50,120-143, each atom re-wrapped), displacements and
+// the prefix of the moved atoms' list rows.
+__device__ void make_trial(const DevSys &s, MoveLDS &m)
+{
+    const int a = threadIdx.x;
+    if (a < m.nm) {
+        const double x = m.ox[a], y = m.oy[a], z = m.oz[a];
         double ux, uy, uz;
         if (m.type == MCX_MV_TRN_DEV) {
             ux = x + m.r[0]; uy = y + m.r[1]; uz = z + m.r[2];
         } else {
-            const double px = x - m.piv[0], py = y - m.piv[1], pz = z - m.piv[2];
-            ux = m.piv[0] + m.R[0][0] * px + m.R[1][0] * py + m.R[2][0] * pz;
-            uy = m.piv[1] + m.R[0][1] * px + m.R[1][1] * py + m.R[2][1] * pz;
-            uz = m.piv[2] + m.R[0][2] * px + m.R[1][2] * py + m.R[2][2] * pz;
+            double piv[3];
+            const int pv = m.pivot;
+            if (pv >= m.first && pv <= m.last) {
+                piv[0] = m.ox[pv - m.first]; piv[1] = m.oy[pv - m.first]; piv[2] = m.oz[pv - m.first];
+            } else if (pv >= 0) {
+                piv[0] = m.xpiv[0]; piv[1] = m.xpiv[1]; piv[2] = m.xpiv[2];
+            } else {   // Tools::getCentreOfMass, include/Tools.hpp:95-113
+                double mt_ = 0.0, c0 = 0.0, c1 = 0.0, c2 = 0.0;
+                for (int b = 0; b < m.nm; b++) {
+                    const double ms = m.ms[b];
+                    mt_ += ms; c0 += ms * m.ox[b]; c1 += ms * m.oy[b]; c2 += ms * m.oz[b];
+                }
+                piv[0] = c0 / mt_; piv[1] = c1 / mt_; piv[2] = c2 / mt_;
+            }
+            const double px = x - piv[0], py = y - piv[1], pz = z - piv[2];
+            ux = piv[0] + m.R[0][0] * px + m.R[1][0] * py + m.R[2][0] * pz;
+            uy = piv[1] + m.R[0][1] * px + m.R[1][1] * py + m.R[2][1] * pz;
+            uz = piv[2] + m.R[0][2] * px + m.R[1][2] * py + m.R[2][2] * pz;
         }
         m.dx[a] = ux - x; m.dy[a] = uy - y; m.dz[a] = uz - z;
         m.tx[a] = wrap1(s, ux, 0); m.ty[a] = wrap1(s, uy, 1); m.tz[a] = wrap1(s, uz, 2);
+    }
+    if (threadIdx.x == 64) {
+        int acc = 0;
+        for (int b = 0; b < m.nm; b++) { m.pref[b] = acc; acc += m.mc[b]; }
+        m.pref[m.nm] = acc;
+        m.total = acc;
     }
     __syncthreads();
 }
@@ -319,107 +450,132 @@ __device__ void make_trial(const DevSys &s, MoveLDS &m)
 __global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s, DevTerms t, DevMoves mv, MCRunArgs arg)
 {
     __shared__ MoveLDS m;
-    __shared__ MT mt;
+    __shared__ uint32_t mt[kMTN];
     __shared__ double dmax[kMaxMoveTypes];
     __shared__ int acc_tmp[kMaxMoveTypes];
-    __shared__ Rng rng;
+    __shared__ long long n_trial[kMaxMoveTypes], n_acc[kMaxMoveTypes];
+    __shared__ double etot;
     const int tid = threadIdx.x;
-    if (tid == 0) {
-        for (int k = 0; k < mv.ntypes; k++) { dmax[k] = arg.dmax[k]; acc_tmp[k] = arg.acc_tmp[k]; }
-        rng.mt = nullptr;
-        rng.key = make_uint2((uint32_t)arg.seed, (uint32_t)(arg.seed >> 32));
-        rng.chain = 0;
-        rng.nbuf = 0;
-        rng.ctr = ((uint64_t)arg.mt_state[624] << 32) | arg.mt_state[625];
-        if (arg.rng_mode == 1) {
-            rng.mt = &mt;
-            mt.idx = (int)arg.mt_state[624];
+    const int lane = tid & 63;
+    const bool w0 = tid < 64;
+    // run state: [0,624) MT state, 624 MT index, 625/626 Philox counter,
+    // 627 cursor into the draw batch, 628.. the batch (32 doubles)
+    WaveDraws dr;
+    if (w0) {
+        dr.replay = arg.rng_mode == 1;
+        dr.mt = mt;
+        dr.mt_idx = (int)arg.mt_state[624];
+        dr.ctr = ((uint64_t)arg.mt_state[625] << 32) | arg.mt_state[626];
+        dr.cur = (int)arg.mt_state[627];
+        dr.key = make_uint2((uint32_t)arg.seed, (uint32_t)(arg.seed >> 32));
+        dr.chain = 0;
+        const uint32_t *bv = arg.mt_state + 628;
+        dr.v = lane < 32 ? __hiloint2double((int)bv[2 * lane + 1], (int)bv[2 * lane]) : 0.0;
+        if (lane < kMaxMoveTypes && lane < mv.ntypes) {
+            dmax[lane] = arg.dmax[lane]; acc_tmp[lane] = arg.acc_tmp[lane];
+            n_trial[lane] = arg.trials[lane]; n_acc[lane] = arg.accepted[lane];
         }
+        if (lane == 0) etot = arg.etot[0];
     }
     if (arg.rng_mode == 1)
-        for (int k = tid; k < 624; k += kMCT) mt.mt[k] = arg.mt_state[k];
+        for (int k = tid; k < kMTN; k += kMCT) mt[k] = arg.mt_state[k];
     __syncthreads();
 
     for (int step = 0; step < arg.nsteps; step++) {
         const int st = arg.step0 + step + 1;
-        if (tid == 0) {
-            const int tp = (int)((double)mv.ntypes * rng.u01());
-            const int in = (int)((double)mv.ninst[tp] * rng.u01());
-            const int g = mv.off[tp] + in;
-            m.type = mv.type[tp];
-            m.inst = in;
-            m.first = mv.first[g];
-            m.last = mv.last[g];
-            m.nm = m.last - m.first + 1;
+        // ---- phase A (wavefront 0, lanes in step): pick and draw the move
+        // (src/MC_metropolis.cpp:85-116; src/MC.cpp:137-156)
+        int tp = 0;
+        if (w0) {
+            tp = (int)((double)mv.ntypes * dr.next(lane));
+            const int in = (int)((double)mv.ninst[tp] * dr.next(lane));
+            const int4 rec = mv.inst[mv.off[tp] + in];
+            const int2 rr = mv.inst_adj[mv.off[tp] + in];
             double r[3];
             do {
-                r[0] = rng.u01() * 2.0 + (-1.0);
-                r[1] = rng.u01() * 2.0 + (-1.0);
-                r[2] = rng.u01() * 2.0 + (-1.0);
+                r[0] = dr.next(lane) * 2.0 + (-1.0);
+                r[1] = dr.next(lane) * 2.0 + (-1.0);
+                r[2] = dr.next(lane) * 2.0 + (-1.0);
             } while ((r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) > 1.0);
-            if (m.type == MCX_MV_TRN_DEV) {
-                m.r[0] = r[0] * dmax[tp]; m.r[1] = r[1] * dmax[tp]; m.r[2] = r[2] * dmax[tp];
+            const int type = mv.type[tp];
+            double R[3][3];
+            if (type == MCX_MV_TRN_DEV) {
+                r[0] = r[0] * dmax[tp]; r[1] = r[1] * dmax[tp]; r[2] = r[2] * dmax[tp];
             } else {
-                const double rang = dmax[tp] * (rng.u01() * 2.0 + (-1.0));
-                rot_mat(m.R, r, rang);
-                const int pv = mv.pivot[g];
-                if (pv >= 0) { m.piv[0] = s.x[pv]; m.piv[1] = s.y[pv]; m.piv[2] = s.z[pv]; }
-                else {
-                    double mt_ = 0.0, c0 = 0.0, c1 = 0.0, c2 = 0.0;
-                    for (int i = m.first; i <= m.last; i++) {
-                        const double ms = mv.mass[i];
-                        mt_ += ms; c0 += ms * s.x[i]; c1 += ms * s.y[i]; c2 += ms * s.z[i];
-                    }
-                    m.piv[0] = c0 / mt_; m.piv[1] = c1 / mt_; m.piv[2] = c2 / mt_;
-                }
+                const double rang = dmax[tp] * (dr.next(lane) * 2.0 + (-1.0));
+                rot_mat(R, r, rang);
             }
-            arg.trials[tp]++;
-            m.acc = tp;   // remember the type until the test
+            if (lane == 0) {
+                m.type = type; m.inst = in; m.acc = tp;
+                m.first = rec.x; m.last = rec.y; m.pivot = rec.z;
+                m.ra0 = rr.x; m.ra1 = rr.y;
+                m.nm = rec.y - rec.x + 1;
+                m.r[0] = r[0]; m.r[1] = r[1]; m.r[2] = r[2];
+                if (type != MCX_MV_TRN_DEV)
+                    for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) m.R[a][b] = R[a][b];
+            }
         }
         __syncthreads();
+        load_moved(s, mv, m);
         make_trial(s, m);
         move_delta(s, t, mv, m);
-        if (tid == 0) {
-            const int tp = m.acc;
+        // ---- phase E+F (wavefront 0): Metropolis test (apply_criterion,
+        // src/MC_metropolis.cpp:175-204), bookkeeping, commit of the moved
+        // atoms; the next phase-A barrier publishes the commit
+        if (w0) {
+            tp = m.acc;
             const double de = m.de;
             int acc;
             if (de <= DBL_EPSILON) acc = 1;
             else {
-                const double alpha = rng.u01();
+                const double alpha = dr.next(lane);
                 acc = alpha < exp(-arg.beta * de);
             }
-            if (acc) { arg.etot[0] += de; arg.accepted[tp]++; acc_tmp[tp]++; }
-            const int o = st - 1;
-            if (arg.acc_out) arg.acc_out[o] = (unsigned char)acc;
-            if (arg.de_out) arg.de_out[o] = de;
-            if (arg.mvtyp_out) arg.mvtyp_out[o] = tp;
-            if (arg.mvinst_out) arg.mvinst_out[o] = m.inst;
-            for (int k = 0; k < mv.ntypes; k++) {
-                if (mv.each[k] != 0 && (st % mv.each[k]) == 0) {
-                    const double ratio = (double)acc_tmp[k] / (double)mv.each[k];
-                    if (ratio > mv.target[k] / 100) dmax[k] *= 1.10;
-                    else dmax[k] *= 0.9;
-                    acc_tmp[k] = 0;
+            if (lane == 0) {
+                n_trial[tp]++;
+                if (acc) { etot += de; n_acc[tp]++; }
+                const int o = st - 1;
+                if (arg.acc_out) arg.acc_out[o] = (unsigned char)acc;
+                if (arg.de_out) arg.de_out[o] = de;
+                if (arg.mvtyp_out) arg.mvtyp_out[o] = tp;
+                if (arg.mvinst_out) arg.mvinst_out[o] = m.inst;
+                if (acc) acc_tmp[tp]++;
+                for (int k = 0; k < mv.ntypes; k++) {
+                    if (mv.each[k] != 0 && (st % mv.each[k]) == 0) {
+                        const double ratio = (double)acc_tmp[k] / (double)mv.each[k];
+                        if (ratio > mv.target[k] / 100) dmax[k] *= 1.10;
+                        else dmax[k] *= 0.9;
+                        acc_tmp[k] = 0;
+                    }
                 }
             }
-            m.acc = acc;
+            if (acc && lane < m.nm) {
+                const int i = m.first + lane;
+                s.x[i] = m.tx[lane]; s.y[i] = m.ty[lane]; s.z[i] = m.tz[lane];
+                s.ipos[i] = mcx_fixed4(s, m.tx[lane], m.ty[lane], m.tz[lane], __int_as_float(m.ip[lane].w));
+            }
         }
-        __syncthreads();
-        if (m.acc && tid < m.nm) {
-            const int i = m.first + tid;
-            s.x[i] = m.tx[tid]; s.y[i] = m.ty[tid]; s.z[i] = m.tz[tid];
-            const float q = __int_as_float(s.ipos[i].w);
-            s.ipos[i] = mcx_fixed4(s, m.tx[tid], m.ty[tid], m.tz[tid], q);
-        }
-        __syncthreads();
     }
-    if (tid == 0) {
-        for (int k = 0; k < mv.ntypes; k++) { arg.dmax[k] = dmax[k]; arg.acc_tmp[k] = acc_tmp[k]; }
-        if (arg.rng_mode == 1) arg.mt_state[624] = (uint32_t)mt.idx;
-        else { arg.mt_state[624] = (uint32_t)(rng.ctr >> 32); arg.mt_state[625] = (uint32_t)rng.ctr; }
+    __syncthreads();
+    if (w0) {
+        if (lane < mv.ntypes) {
+            arg.dmax[lane] = dmax[lane]; arg.acc_tmp[lane] = acc_tmp[lane];
+            arg.trials[lane] = n_trial[lane]; arg.accepted[lane] = n_acc[lane];
+        }
+        if (lane == 0) {
+            arg.etot[0] = etot;
+            arg.mt_state[624] = (uint32_t)dr.mt_idx;
+            arg.mt_state[625] = (uint32_t)(dr.ctr >> 32);
+            arg.mt_state[626] = (uint32_t)dr.ctr;
+            arg.mt_state[627] = (uint32_t)dr.cur;
+        }
+        if (lane < 32) {
+            arg.mt_state[628 + 2 * lane] = (uint32_t)__double2loint(dr.v);
+            arg.mt_state[628 + 2 * lane + 1] = (uint32_t)__double2hiint(dr.v);
+        }
     }
     if (arg.rng_mode == 1)
-        for (int k = tid; k < 624; k += kMCT) arg.mt_state[k] = mt.mt[k];
+        for (int k = tid; k < kMTN; k += kMCT) arg.mt_state[k] = mt[k];
 }
 
 // one-shot dE for caller-supplied trial coordinates
@@ -428,14 +584,25 @@ __global__ __launch_bounds__(kMCT) void k_delta(DevSys s, DevTerms t, DevMoves m
 {
     __shared__ MoveLDS m;
     const int nm = last - first + 1;
-    if (threadIdx.x == 0) { m.first = first; m.last = last; m.nm = nm; }
+    if (threadIdx.x == 0) {
+        m.first = first; m.last = last; m.nm = nm; m.pivot = -1; m.type = MCX_MV_TRN_DEV;
+        m.ra0 = 0; m.ra1 = -1;
+    }
+    __syncthreads();
+    load_moved(s, mv, m);
     if (threadIdx.x < nm) {
-        const int a = threadIdx.x, i = first + a;
+        const int a = threadIdx.x;
         const double nx = nxyz[3 * a], ny = nxyz[3 * a + 1], nz = nxyz[3 * a + 2];
-        double d[3] = {nx - s.x[i], ny - s.y[i], nz - s.z[i]};
+        double d[3] = {nx - m.ox[a], ny - m.oy[a], nz - m.oz[a]};
         mcx_pbc(s, d);
         m.dx[a] = d[0]; m.dy[a] = d[1]; m.dz[a] = d[2];
         m.tx[a] = nx; m.ty[a] = ny; m.tz[a] = nz;
+    }
+    if (threadIdx.x == 64) {
+        int acc = 0;
+        for (int b = 0; b < nm; b++) { m.pref[b] = acc; acc += m.mc[b]; }
+        m.pref[nm] = acc;
+        m.total = acc;
     }
     __syncthreads();
     move_delta(s, t, mv, m);

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <memory>
 
 namespace {
@@ -62,7 +63,8 @@ struct mcx_ctx {
     DBuf<int4> ipos;
     DBuf<float2> es, esk, es14;
     std::vector<double> hx, hy, hz;   // host copy of the last uploaded coordinates
-    DBuf<int> frozen_d, excl_off_d, excl_idx_d;
+    DBuf<int> frozen_d, excl_off_d, excl_idx_d, xfull_off_d, xfull_idx_d;
+    DBuf<double4> cpos;
     DBuf<int> nbr, cnt, half, overflow;
     int cap = 0;
     DBuf<int> cell_of, cell_start, cell_count, cell_atoms, cursor;
@@ -73,8 +75,15 @@ struct mcx_ctx {
     DBuf<int4> t_idx[kNumKinds];
     DBuf<int> t_type[kNumKinds];
     DBuf<double4> t_par[kNumKinds];
-    DBuf<int> adj_off, adj_term;
-    DBuf<int> mv_first, mv_last, mv_pivot;
+    DBuf<int> adj_off, adj_kk, adj_type;
+    DBuf<int4> adj_ids;
+    DBuf<double4> adj_par;
+    std::vector<int> adj_off_h, adj_kk_h;
+    DBuf<int4> mv_rid;
+    DBuf<int> mv_rkk, mv_rty;
+    DBuf<double4> mv_rpar;
+    DBuf<int2> mv_inst_adj;
+    DBuf<int4> mv_inst;
     DBuf<double> newxyz, scalar;
     long long last_npairs = 0;
 
@@ -104,6 +113,7 @@ struct mcx_ctx {
         s.q64 = q64.p; s.eps64 = eps64.p; s.sig64 = sig64.p; s.eps1464 = eps1464.p; s.sig1464 = sig1464.p;
         s.frozen = frozen_d.p;
         s.excl_off = excl_off_d.p; s.excl_idx = excl_idx_d.p;
+        s.xfull_off = xfull_off_d.p; s.xfull_idx = xfull_idx_d.p;
         s.cap = cap;
         s.nbr = nbr.p; s.cnt = cnt.p; s.half = half.p; s.overflow = overflow.p;
         return s;
@@ -134,9 +144,15 @@ struct mcx_ctx {
                              &newxyz, &scalar};
         for (auto *b : d) b->release();
         ipos.release(); es.release(); esk.release(); es14.release();
-        DBuf<int> *ib[] = {&frozen_d, &excl_off_d, &excl_idx_d, &nbr, &cnt, &half, &overflow, &cell_of,
-                           &cell_start, &cell_count, &cell_atoms, &cursor, &adj_off, &adj_term, &mv_first,
-                           &mv_last, &mv_pivot};
+        cpos.release();
+        adj_ids.release();
+        adj_par.release();
+        adj_type.release();
+        mv_inst_adj.release();
+        mv_rid.release(); mv_rkk.release(); mv_rty.release(); mv_rpar.release();
+        mv_inst.release();
+        DBuf<int> *ib[] = {&frozen_d, &excl_off_d, &excl_idx_d, &xfull_off_d, &xfull_idx_d, &nbr, &cnt, &half, &overflow, &cell_of,
+                           &cell_start, &cell_count, &cell_atoms, &cursor, &adj_off, &adj_kk};
         for (auto *b : ib) b->release();
         tiles.release();
         for (int k = 0; k < kNumKinds; k++) { t_idx[k].release(); t_type[k].release(); t_par[k].release(); }
@@ -268,10 +284,10 @@ void upload_terms(mcx_ctx *h)
         h->t_type[k].upload(h->terms.type[k].data(), h->terms.type[k].size(), h->stream);
         h->t_par[k].upload(h->terms.par[k].data(), h->terms.par[k].size(), h->stream);
     }
-    // per-atom adjacency of bonded terms and 1-4 pairs (global work ids)
+    // per-atom adjacency of bonded terms and 1-4 pairs: atom ids inline and
+    // (kind << 28 | index) so that the dE kernel needs no indirection
     const int N = h->n;
-    std::vector<std::vector<int>> adj(N);
-    int base = 0;
+    std::vector<std::vector<std::pair<int4, int>>> adj(N);
     for (int k = 0; k <= kPair14; k++) {
         const auto &v = h->terms.idx[k];
         for (size_t t = 0; t < v.size(); t++) {
@@ -281,20 +297,51 @@ void upload_terms(mcx_ctx *h)
                 if (a < 0) continue;
                 bool dup = false;
                 for (int p = 0; p < q; p++) dup |= ids[p] == a;
-                if (!dup) adj[a].push_back(base + (int)t);
+                if (!dup) adj[a].push_back(std::make_pair(v[t], (k << 28) | (int)t));
             }
         }
-        base += (int)v.size();
     }
-    std::vector<int> off(N + 1, 0), flat;
+    std::vector<int> off(N + 1, 0), kk, ty;
+    std::vector<int4> ids;
+    std::vector<double4> par;
     for (int i = 0; i < N; i++) {
         off[i + 1] = off[i] + (int)adj[i].size();
-        flat.insert(flat.end(), adj[i].begin(), adj[i].end());
+        for (const auto &p : adj[i]) {
+            ids.push_back(p.first);
+            kk.push_back(p.second);
+            const int kind = p.second >> 28, k = p.second & 0x0fffffff;
+            ty.push_back(h->terms.type[kind][k]);
+            par.push_back(h->terms.par[kind][k]);
+        }
     }
+    h->adj_off_h = off;
+    h->adj_kk_h = kk;
     h->adj_off.upload(off.data(), off.size(), h->stream);
-    h->adj_term.upload(flat.data(), flat.size(), h->stream);
+    h->adj_ids.upload(ids.data(), ids.size(), h->stream);
+    h->adj_kk.upload(kk.data(), kk.size(), h->stream);
+    h->adj_type.upload(ty.data(), ty.size(), h->stream);
+    h->adj_par.upload(par.data(), par.size(), h->stream);
     h->excl_off_d.upload(h->excl_off.data(), h->excl_off.size(), h->stream);
     h->excl_idx_d.upload(h->excl_idx.data(), h->excl_idx.size(), h->stream);
+    // both-direction excluded partners (pair (a,b), a<b, excluded iff b is
+    // in the visible row of a), used by the list build to clear bits
+    std::vector<std::vector<int>> xf(N);
+    for (int a = 0; a < N; a++)
+        for (int k = h->excl_off[a]; k < h->excl_off[a + 1]; k++) {
+            const int b = h->excl_idx[k];
+            if (b <= a) continue;
+            xf[a].push_back(b);
+            xf[b].push_back(a);
+        }
+    std::vector<int> xo(N + 1, 0), xi;
+    for (int a = 0; a < N; a++) {
+        std::sort(xf[a].begin(), xf[a].end());
+        xf[a].erase(std::unique(xf[a].begin(), xf[a].end()), xf[a].end());
+        xo[a + 1] = xo[a] + (int)xf[a].size();
+        xi.insert(xi.end(), xf[a].begin(), xf[a].end());
+    }
+    h->xfull_off_d.upload(xo.data(), xo.size(), h->stream);
+    h->xfull_idx_d.upload(xi.data(), xi.size(), h->stream);
 }
 
 void setup_grid(mcx_ctx *h)
@@ -315,6 +362,8 @@ void setup_grid(mcx_ctx *h)
     h->cell_start.alloc(g.ncell);
     h->cell_count.alloc(g.ncell);
     h->cursor.alloc(g.ncell);
+    h->cpos.alloc(h->n);
+    g.cpos = h->cpos.p;
     g.cell_of = h->cell_of.p;
     g.cell_atoms = h->cell_atoms.p;
     g.cell_start = h->cell_start.p;
@@ -336,7 +385,7 @@ void ensure_list_capacity(mcx_ctx *h, int cap)
 long long build_list(mcx_ctx *h)
 {
     if (!h->have_coords || !h->have_excl) throw mcx_error(MCX_ERR_STATE, "coordinates and exclusions must be set before the list");
-    if ((size_t)((h->n + 31) / 32) * 4 > 160 * 1024) throw mcx_error(MCX_ERR_UNSUPPORTED, "list build supports up to 1.3M atoms");
+    if ((size_t)((h->n + 31) / 32) * 4 > 60 * 1024) throw mcx_error(MCX_ERR_UNSUPPORTED, "list build supports up to 491,520 atoms");
     setup_grid(h);
     if (h->cap == 0) {
         double vol = h->box[0] * h->box[1] * h->box[2];
@@ -552,13 +601,54 @@ DevMoves build_moves(mcx_ctx *h, const mcx_move_type *types, int ntypes)
         if (m.mode == MCX_MODE_ALL && h->n > kMaxMoveAtoms)
             throw mcx_error(MCX_ERR_UNSUPPORTED, "ALL-mode moves of more than 64 atoms");
     }
-    h->mv_first.upload(F.data(), F.size(), h->stream);
-    h->mv_last.upload(L.data(), L.size(), h->stream);
-    h->mv_pivot.upload(P.data(), P.size(), h->stream);
-    mv.first = h->mv_first.p; mv.last = h->mv_last.p; mv.pivot = h->mv_pivot.p;
+    // every bonded term / 1-4 pair touching an instance's atom range, once,
+    // as self-contained records (atoms, kind|index, type, parameters);
+    // instances with the same range share one record list
+    std::vector<int4> rec(F.size());
+    std::vector<int2> radj(F.size());
+    std::map<std::pair<int, int>, int2> seen;
+    std::vector<int4> rid;
+    std::vector<int> rkk, rty;
+    std::vector<double4> rpar;
+    for (size_t k = 0; k < F.size(); k++) {
+        rec[k] = make_int4(F[k], L[k], P[k], 0);
+        const auto key = std::make_pair(F[k], L[k]);
+        auto it = seen.find(key);
+        if (it != seen.end()) { radj[k] = it->second; continue; }
+        const int begin = (int)rid.size();
+        std::vector<int> used;
+        for (int a = F[k]; a <= L[k]; a++)
+            for (int e = h->adj_off_h[a]; e < h->adj_off_h[a + 1]; e++) {
+                const int kkv = h->adj_kk_h[e];
+                if (std::find(used.begin(), used.end(), kkv) != used.end()) continue;
+                used.push_back(kkv);
+                const int kind = kkv >> 28, ti = kkv & 0x0fffffff;
+                rid.push_back(h->terms.idx[kind][ti]);
+                rkk.push_back(kkv);
+                rty.push_back(h->terms.type[kind][ti]);
+                rpar.push_back(h->terms.par[kind][ti]);
+            }
+        radj[k] = make_int2(begin, (int)rid.size());
+        seen[key] = radj[k];
+    }
+    h->mv_rid.upload(rid.data(), rid.size(), h->stream);
+    h->mv_rkk.upload(rkk.data(), rkk.size(), h->stream);
+    h->mv_rty.upload(rty.data(), rty.size(), h->stream);
+    h->mv_rpar.upload(rpar.data(), rpar.size(), h->stream);
+    h->mv_inst.upload(rec.data(), rec.size(), h->stream);
+    h->mv_inst_adj.upload(radj.data(), radj.size(), h->stream);
+    mv.inst = h->mv_inst.p;
+    mv.inst_adj = h->mv_inst_adj.p;
+    mv.rec_ids = h->mv_rid.p;
+    mv.rec_kk = h->mv_rkk.p;
+    mv.rec_type = h->mv_rty.p;
+    mv.rec_par = h->mv_rpar.p;
     mv.mass = h->mass64.p;
     mv.adj_off = h->adj_off.p;
-    mv.adj_term = h->adj_term.p;
+    mv.adj_ids = h->adj_ids.p;
+    mv.adj_kk = h->adj_kk.p;
+    mv.adj_type = h->adj_type.p;
+    mv.adj_par = h->adj_par.p;
     return mv;
 }
 
@@ -566,7 +656,10 @@ DevMoves adjacency_only(mcx_ctx *h)
 {
     DevMoves mv{};
     mv.adj_off = h->adj_off.p;
-    mv.adj_term = h->adj_term.p;
+    mv.adj_ids = h->adj_ids.p;
+    mv.adj_kk = h->adj_kk.p;
+    mv.adj_type = h->adj_type.p;
+    mv.adj_par = h->adj_par.p;
     mv.mass = h->mass64.p;
     return mv;
 }
@@ -576,7 +669,6 @@ void mt_seed(uint32_t *st, uint64_t seed)
     st[0] = (uint32_t)(seed & 0xffffffffu);
     for (int i = 1; i < 624; i++) st[i] = 1812433253u * (st[i - 1] ^ (st[i - 1] >> 30)) + (uint32_t)i;
     st[624] = 624;
-    st[625] = 0;
 }
 
 }  // namespace
@@ -892,9 +984,10 @@ int mcx_mc_run(mcx_ctx *h, const mcx_move_type *types, int ntypes, int nsteps, u
         std::vector<double> dm(ntypes);
         for (int t = 0; t < ntypes; t++) dm[t] = types[t].dmax;
         dmax.upload(dm.data(), ntypes, h->stream);
-        std::vector<uint32_t> st(626);
+        std::vector<uint32_t> st(692, 0);
         mt_seed(st.data(), seed);
-        if (rng_mode != MCX_MC_REPLAY) { st[624] = 0; st[625] = 0; }
+        st[625] = 0; st[626] = 0;
+        st[627] = 32;     // empty draw batch: the first draw refills it
         mts.upload(st.data(), st.size(), h->stream);
         std::vector<long long> z8(ntypes, 0);
         trials.upload(z8.data(), ntypes, h->stream);

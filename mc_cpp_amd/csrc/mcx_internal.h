@@ -53,6 +53,9 @@ struct DevSys {
     const int *frozen;
     // exclusion rows (visible entries, CSR)
     const int *excl_off, *excl_idx;
+    // excluded partners of each atom in both directions (pair (a,b), a<b,
+    // is excluded iff b is in the visible row of a), CSR
+    const int *xfull_off, *xfull_idx;
     // neighbour list: full rows (all j != i, ascending) in ELL layout
     int cap;
     int *nbr;            // [n*cap]
@@ -82,10 +85,21 @@ struct DevMoves {
     double dmax0[kMaxMoveTypes];
     double target[kMaxMoveTypes];
     int each[kMaxMoveTypes];
-    const int *first, *last, *pivot;   // flat instance tables
+    const int4 *inst;                  // flat instance table: (first, last, pivot, 0)
+    const int2 *inst_adj;              // range of the instance's term records
+    // term records: every bonded term / 1-4 pair touching an instance, once
+    const int4 *rec_ids;
+    const int *rec_kk;                 // kind << 28 | index within kind
+    const int *rec_type;
+    const double4 *rec_par;
     const double *mass;                // for centre-of-mass pivots
-    // per-atom adjacency of bonded terms (CSR over atoms -> global term ids)
-    const int *adj_off, *adj_term;
+    // per-atom adjacency of bonded terms and 1-4 pairs (CSR over atoms):
+    // the term's atoms and (kind << 28 | index within kind)
+    const int *adj_off;
+    const int4 *adj_ids;
+    const int *adj_kk;                 // (kind << 28 | index), bit 27.. of index unused
+    const int *adj_type;
+    const double4 *adj_par;
 };
 
 struct HostTerms {
@@ -120,6 +134,7 @@ struct CellGrid {
     int ncell;
     int reach[3];            // cells searched in each direction
     int *cell_of, *cell_start, *cell_count, *cell_atoms, *cursor;
+    double4 *cpos;           // cell-ordered (x, y, z, 2*atom + frozen)
 };
 void launch_list_build(const DevSys &s, const CellGrid &g, hipStream_t st);
 size_t list_build_lds_bytes(int n);
@@ -135,7 +150,7 @@ struct MCRunArgs {
     uint64_t seed;
     double beta;
     double *etot;          // device scalar
-    uint32_t *mt_state;    // [626] device: MT19937 state + index, or the Philox counter in [624..625]
+    uint32_t *mt_state;    // [692] device: MT19937 state, index, Philox counter, draw batch
     double *dmax;          // [ntypes] device
     long long *trials, *accepted;   // [ntypes] device
     int *acc_tmp;          // [ntypes]

@@ -165,17 +165,15 @@ struct CoordFetch {
 // vdw_raw = e_i e_j (t^6 - t^3) * S, to be scaled by 332.0716*418.4 and 4).
 // Bonded formulas restate src/FField_MDBAS.cpp:490-786 operation by
 // operation in FP64.
-__device__ inline void mcx_term_energy(const DevSys &s, const DevTerms &t, int kind, int k, const CoordFetch &f,
-                                       double &e, double &eel, double &evdw)
+__device__ inline void mcx_term_energy_rec(const DevSys &s, int kind, const int4 id, int type, const double4 p,
+                                           const CoordFetch &f, double &e, double &eel, double &evdw)
 {
     e = 0.0; eel = 0.0; evdw = 0.0;
-    const int4 id = t.idx[kind][k];
     if (kind == kBond || kind == kUB) {
         double di[3], dj[3], d[3];
         f.get(id.x, di); f.get(id.y, dj);
         const double r = sqrt(mcx_dist2(s, di, dj, d));
-        const double4 p = t.par[kind][k];
-        if (kind == kBond && t.type[kind][k] == 1) {
+        if (kind == kBond && type == 1) {
             const double ma = exp(-p.z * (r - p.y));
             const double mb = ma * ma;
             e = p.x * (mb - 2. * ma) + p.x;
@@ -190,7 +188,6 @@ __device__ inline void mcx_term_energy(const DevSys &s, const DevTerms &t, int k
         const double rbc = sqrt(mcx_dist2(s, dk, dj, dbc));
         const double cost = (dab[0] * dbc[0] + dab[1] * dbc[1] + dab[2] * dbc[2]) / (rab * rbc);
         const double th = acos(cost);
-        const double4 p = t.par[kind][k];
         const double x = th - p.y;
         e = 0.5 * p.x * (x * x);
     } else if (kind == kDihe || kind == kImpr) {
@@ -214,8 +211,7 @@ __device__ inline void mcx_term_energy(const DevSys &s, const DevTerms &t, int k
         const double sinp = (dbc[0] * (pc[1] * pb[2] - pc[2] * pb[1]) + dbc[1] * (pb[0] * pc[2] - pb[2] * pc[0]) +
                              dbc[2] * (pc[0] * pb[1] - pc[1] * pb[0])) / (rpb * rpc * rbc);
         double phi = atan2(sinp, cosp);
-        const double4 p = t.par[kind][k];
-        if (t.type[kind][k] == 2) {
+        if (type == 2) {
             phi = phi - p.y;
             phi = phi - mcx_ref_rint(phi / MCX_TWOPI) * MCX_TWOPI;
             e = 0.5 * p.x * (phi * phi);
@@ -255,4 +251,10 @@ __device__ inline void mcx_term_energy(const DevSys &s, const DevTerms &t, int k
         eel = sw * ((double)(float)s.q64[id.x] * c);
         evdw = sw * ((double)ei.x * (double)v);
     }
+}
+
+__device__ inline void mcx_term_energy(const DevSys &s, const DevTerms &t, int kind, int k, const CoordFetch &f,
+                                       double &e, double &eel, double &evdw)
+{
+    mcx_term_energy_rec(s, kind, t.idx[kind][k], t.type[kind][k], t.par[kind][k], f, e, eel, evdw);
 }

@@ -156,6 +156,14 @@ def _compare_replay(res, ro, beta, tol_fn):
     print("first steps (typ gpu/fp64, inst gpu/fp64, dE gpu/fp64):", head)
     mism = np.nonzero(acc_g != acc_o)[0]
     n_ok = len(acc_g) if len(mism) == 0 else int(mism[0])
+    bad = np.nonzero((res["mvinst"] != ro["mvinst"]) | (res["mvtyp"] != ro["mvtyp"]))[0]
+    if len(bad):
+        print("first differing move pick at step", int(bad[0]))
+    dd = np.nonzero(np.abs(res["de"] - ro["de"]) > np.array([tol_fn(v) for v in ro["de"]]))[0]
+    if len(dd):
+        k = int(dd[0])
+        print("first dE outside tolerance at step", k, "pick gpu", int(res["mvtyp"][k]), int(res["mvinst"][k]),
+              "fp64", int(ro["mvtyp"][k]), int(ro["mvinst"][k]), "acc before", int(acc_g[:k].sum()), int(acc_o[:k].sum()))
     # dE agreement on the common prefix
     for k in range(n_ok):
         assert abs(res["de"][k] - ro["de"][k]) <= tol_fn(ro["de"][k]), (k, res["de"][k], ro["de"][k])
