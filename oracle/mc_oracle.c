@@ -524,6 +524,9 @@ void orc_eimpr(orc_system *s)
 /* getE(double[10]) / getEtot / getEswitch, src/FField_MDBAS.cpp:77-192 */
 double orc_getE(orc_system *s, double ener[10])
 {
+    /* the reference builds the list in the List_nonBonded constructor
+     * (src/List_nonBonded.cpp:62-100) before any getE */
+    if (s->cutmode == ORC_CUT_SWITCH && s->neighPair == NULL) orc_update_verlet_list(s, 1);
     if (s->cutmode == ORC_CUT_FULL) { orc_nonbonded_full(s); orc_nonbonded14(s); }
     else { orc_nonbonded_switch(s); orc_nonbonded14_switch(s); }
     if (s->nbond > 0) orc_ebond(s);

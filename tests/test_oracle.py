@@ -1,0 +1,316 @@
+"""Pins the FP64 oracle (oracle/mc_oracle.c) -- CPU only.
+
+* against the reference's own outputs (tests/golden/golden.json, produced by
+  the mc_cpp sources compiled in oracle/_ref): energies and pair lists must
+  agree bit for bit, Metropolis runs must end on the same accept counts,
+  dmax and final coordinates;
+* against mathematics the formulas fix: known answers of MT19937 and
+  Philox4x32-10, closed-form pair energies, switch end points, brute-force
+  sums and pair sets on small boxes, exclusion combinatorics, bonded terms
+  at their reference geometry, rigid-body invariants of the moves.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from mc_cpp_amd import philox, systems
+from oracle import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "golden.json")))
+REF_TEST = "/root/reference/test"
+KCAL = 418.4
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def list_digest(o):
+    cnt, off, flat = o.list_csr()
+    return {"npairs": int(cnt.sum()), "counts_sha256": sha(cnt.astype(np.int32)), "flat_sha256": sha(flat)}
+
+
+# ---------------------------------------------------------------- known answers
+
+def test_mt19937_known_answer(oracle_lib):
+    g = O.MT19937(5489)
+    for _ in range(9999):
+        g.next()
+    assert g.next() == 4123659995      # C++ [rand.predef]: 10000th output of default mt19937
+
+
+def test_generate_canonical_formula(oracle_lib):
+    a, b = O.MT19937(42), O.MT19937(42)
+    for _ in range(100):
+        u1, u2 = b.next(), b.next()
+        assert a.canonical() == (float(u1) + float(u2) * 4294967296.0) / 18446744073709551616.0
+
+
+def test_philox_known_answers():
+    k = philox.philox4x32_10
+    assert [int(v) for v in k(np.zeros((1, 4), np.uint32), np.zeros(2, np.uint32))[0]] == \
+        [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    assert [int(v) for v in k(np.full((1, 4), 0xffffffff, np.uint32), np.full(2, 0xffffffff, np.uint32))[0]] == \
+        [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]
+    assert [int(v) for v in k(np.array([[0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344]], np.uint32),
+                              np.array([0xa4093822, 0x299f31d0], np.uint32))[0]] == \
+        [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+
+
+def test_reference_rint_is_truncation_after_half(oracle_lib):
+    assert O.rint(0.5) == 1.0 and O.rint(-0.5) == -1.0 and O.rint(2.5) == 3.0   # not half-to-even
+    x = 0.49999999999999994                      # x + 0.5 rounds to 1.0 in FP64
+    assert O.rint(x) == 1.0 and round(x) == 0
+
+
+# ---------------------------------------------------------------- reference goldens
+
+def _ref_system(name):
+    g = GOLD[name]
+    if name.startswith("diala"):
+        cor, ff = os.path.join(HERE, "golden", "diala_vac", "ala2.cor"), os.path.join(HERE, "golden", "diala_vac", "ff.dat")
+    elif name == "argon_box":
+        cor, ff = os.path.join(HERE, "golden", "argon_box", "ar.cor"), os.path.join(HERE, "golden", "argon_box", "forfield.dat")
+    else:
+        d = os.path.join(REF_TEST, name)
+        if not os.path.isdir(d):
+            pytest.skip("reference test data not present")
+        cor = [os.path.join(d, f) for f in os.listdir(d) if f.endswith(".cor")][0]
+        ff = os.path.join(d, "forfield.dat")
+    assert hashlib.sha256(open(cor, "rb").read()).hexdigest() == g["cor_sha256"]
+    return systems.read_mdbas(cor, ff, g["box"], pbtype=g["pbtype"], cutmode=g["cutmode"]), g
+
+
+@pytest.mark.parametrize("name", ["diala_vac", "diala_vac_switch", "argon_box", "nma_solvated", "trhbn_solvated"])
+def test_reference_test_systems_bit_exact(oracle_lib, name):
+    s, g = _ref_system(name)
+    o = O.Oracle(s)
+    if s.cutmode == systems.CUT_SWITCH:
+        o.update_list()
+        assert list_digest(o) == g["list"]
+    assert o.getE().tolist() == g["energy"]           # bit for bit
+    rows = o.excl_rows()
+    assert sha(np.array([len(x) for x in rows], np.int32)) == g["excl_counts_sha256"]
+    assert sha(np.concatenate([x for x in rows if len(x)] or [np.zeros(0, np.int32)]).astype(np.int32)) == \
+        g["excl_flat_sha256"]
+
+
+@pytest.mark.parametrize("name", ["argon_fcc_seed1", "water512_seed3", "water4096_seed2"])
+def test_synthetic_systems_bit_exact(oracle_lib, name):
+    gen = {"argon_fcc_seed1": lambda: systems.argon_fcc(seed=1),
+           "water512_seed3": lambda: systems.water_box(seed=3, nside=8, L=24.85),
+           "water4096_seed2": lambda: systems.water_box(seed=2)}[name]
+    s, _ = gen()
+    g = GOLD[name]
+    o = O.Oracle(s)
+    o.update_list()
+    assert o.getE().tolist() == g["energy"]
+    assert list_digest(o) == g["list"]
+    s2 = s.copy()
+    s2.cutmode = systems.CUT_FULL
+    assert O.Oracle(s2).getE().tolist() == g["energy_full"]
+
+
+@pytest.mark.parametrize("name", ["argon_fcc_seed1", "water512_seed3"])
+def test_metropolis_run_matches_reference(oracle_lib, name):
+    """Accept counts, final dmax and final coordinates of MC_metropolis::run."""
+    gen = {"argon_fcc_seed1": lambda: systems.argon_fcc(seed=1),
+           "water512_seed3": lambda: systems.water_box(seed=3, nside=8, L=24.85)}[name]
+    s, _ = gen()
+    g = GOLD[name]["mc"]
+    o = O.Oracle(s)
+    o.update_list()
+    res = o.mc_run(g["nsteps"], g["seed"], mode=1)     # moving-set dE; same decisions as two getE
+    trials = np.bincount(res["mvtyp"], minlength=len(g["trials"]))
+    acc = np.bincount(res["mvtyp"], weights=res["acc"], minlength=len(g["trials"])).astype(int)
+    assert trials.tolist() == g["trials"] and acc.tolist() == g["accepted"]
+    assert res["dmax"].tolist() == g["dmax"]
+    assert sha(np.stack([s.x, s.y, s.z], 1)) == g["final_xyz_sha256"]
+
+
+def test_moving_set_mode_equals_two_full_energies(oracle_lib):
+    s, _ = systems.water_box(seed=5, nside=6, L=18.64)
+    a = s.copy()
+    oa, ob = O.Oracle(s), O.Oracle(a)
+    oa.update_list(); ob.update_list()
+    ra = oa.mc_run(120, 99, mode=0)
+    rb = ob.mc_run(120, 99, mode=1)
+    assert np.array_equal(ra["acc"], rb["acc"])
+    # two full FP64 totals differ from the moving-set sum by their own rounding (~1e-12 of E_tot)
+    assert np.max(np.abs(ra["de"] - rb["de"])) < 1e-11 * abs(ra["etot"])
+
+
+def test_residue_range_last_atom_quirk(oracle_lib):
+    """gtrsfl's inclusive bounds (src/List_Moves.cpp:664): the instance of
+    the last atom of residue 103 resolves to residue 104 -- the reference's
+    final coordinates in the water512 golden run depend on it."""
+    s, _ = systems.water_box(seed=3, nside=8, L=24.85)
+    s.moves = s.moves[:1]
+    o = O.Oracle(s)
+    o.update_list()
+    x0 = s.x.copy()
+    r = o.mc_run(1, 777, mode=0)
+    assert r["mvinst"][0] == 308 and r["acc"][0] == 1
+    assert np.nonzero(s.x != x0)[0].tolist() == [309, 310, 311]
+
+
+# ---------------------------------------------------------------- closed forms
+
+def _two(q1, q2, r, eps=0.238, sig=3.405, L=200.0, cut=systems.CUT_FULL):
+    s = systems.System(natom=2, box=[L, L, L], cutmode=cut, x=[0.0, r], y=[0.0, 0.0], z=[0.0, 0.0],
+                       charge=[q1, q2], eps=[np.sqrt(eps * KCAL)] * 2, sig=[sig / 2] * 2,
+                       eps14=[np.sqrt(eps * KCAL)] * 2, sig14=[sig / 2] * 2)
+    o = O.Oracle(s)
+    if cut == systems.CUT_SWITCH:
+        o.update_list()
+    return o.getE()
+
+
+def test_lj_closed_forms(oracle_lib):
+    sig, eps = 3.405, 0.238
+    rmin = 2 ** (1 / 6) * sig
+    assert _two(0, 0, rmin)[4] == pytest.approx(-eps * KCAL, rel=1e-13)
+    assert abs(_two(0, 0, sig)[4]) < 1e-12
+    r = 4.7
+    assert _two(0, 0, r)[4] == pytest.approx(4 * eps * KCAL * ((sig / r) ** 12 - (sig / r) ** 6), rel=1e-13)
+
+
+def test_coulomb_closed_form(oracle_lib):
+    e = _two(0.5, -0.8, 3.3, eps=0.0)
+    assert e[3] == pytest.approx(332.0716 * KCAL * 0.5 * -0.8 / 3.3, rel=1e-14)
+    assert e[4] == 0.0
+
+
+def test_switch_end_points_and_continuity(oracle_lib):
+    full = lambda r: _two(0.4, -0.4, r, cut=systems.CUT_FULL)[3]
+    sw = lambda r: _two(0.4, -0.4, r, cut=systems.CUT_SWITCH, L=60.0)[3]
+    assert sw(10.0) == pytest.approx(full(10.0), rel=1e-14)       # S(r_on) = 1
+    assert sw(9.0) == pytest.approx(full(9.0), rel=1e-14)
+    assert abs(sw(12.0)) < 1e-20                                  # S(r_off) = 0
+    assert sw(12.5) == 0.0
+    for r0 in (10.0, 12.0):                                      # C1 at both ends: the jump of the
+        jumps = []                                               # one-sided slopes vanishes with h
+        for h in (1e-3, 1e-4):
+            d_in = (sw(r0) - sw(r0 - h)) / h
+            d_out = (sw(r0 + h) - sw(r0)) / h
+            jumps.append(abs(d_in - d_out))
+        assert jumps[1] < 0.2 * jumps[0] + 1e-12
+
+
+def _brute_energy(s):
+    n, L = s.natom, s.box
+    x = np.stack([s.x, s.y, s.z], 1)
+    ec = ev = 0.0
+    for i in range(n - 1):
+        d = x[i + 1:] - x[i]
+        d -= L * np.where(d / L >= 0, np.trunc(d / L + 0.5), np.trunc(d / L - 0.5))
+        r = np.sqrt((d * d).sum(1))
+        ec += (s.charge[i] * s.charge[i + 1:] / r).sum()
+        u = (s.sig[i] + s.sig[i + 1:]) / r
+        ev += (s.eps[i] * s.eps[i + 1:] * (u ** 12 - u ** 6)).sum()
+    return 332.0716 * KCAL * ec, 4 * ev
+
+
+def test_full_energy_brute_force_and_translation(oracle_lib):
+    rng = np.random.default_rng(7)
+    n, L = 60, 15.0
+    s = systems.System(natom=n, box=[L, L, L], cutmode=systems.CUT_FULL,
+                       x=rng.uniform(-L / 2, L / 2, n), y=rng.uniform(-L / 2, L / 2, n), z=rng.uniform(-L / 2, L / 2, n),
+                       charge=rng.uniform(-0.5, 0.5, n), eps=rng.uniform(5, 10, n), sig=rng.uniform(0.5, 0.9, n))
+    o = O.Oracle(s)
+    e = o.getE()
+    ec, ev = _brute_energy(s)
+    assert e[3] == pytest.approx(ec, rel=1e-11) and e[4] == pytest.approx(ev, rel=1e-11)
+    t = s.copy()
+    t.x, t.y, t.z = systems.wrap(t, t.x + 3.3, t.y - 7.1, t.z + 11.9)
+    assert O.Oracle(t).getE()[0] == pytest.approx(e[0], rel=1e-10)
+
+
+def test_pair_set_equals_brute_force(oracle_lib):
+    rng = np.random.default_rng(3)
+    n, L = 300, 31.0
+    s = systems.System(natom=n, box=[L, L, L], x=rng.uniform(-L, L, n), y=rng.uniform(-L, L, n),
+                       z=rng.uniform(-L, L, n))
+    s.frozen[:40] = 1
+    o = O.Oracle(s)
+    o.update_list()
+    cnt, rows = o.list_rows()
+    x = np.stack([s.x, s.y, s.z], 1)
+    for i in range(n):
+        d = x[i + 1:] - x[i]
+        r = d * (1.0 / L)
+        d = d - L * np.where(r >= 0, np.trunc(r + 0.5), np.trunc(r - 0.5))
+        r2 = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]
+        j = np.nonzero(r2 <= 14.0 ** 2)[0] + i + 1
+        j = j[~(s.frozen[i] & s.frozen[j]).astype(bool)]
+        assert np.array_equal(rows[i], j)
+
+
+def test_exclusion_combinatorics(oracle_lib):
+    s, _ = systems.water_box(seed=1, nside=2, L=6.2)
+    rows = O.Oracle(s).excl_rows()
+    for m in range(8):
+        assert list(rows[3 * m]) == [3 * m + 1, 3 * m + 2] and list(rows[3 * m + 1]) == [3 * m + 2]
+    # a linear chain of n beads with bonds, angles, dihedrals
+    n = 40
+    s = systems.System(natom=n, box=[200.0] * 3, x=np.arange(n) * 1.5, y=np.zeros(n), z=np.zeros(n))
+    i = np.arange(n)
+    s.bonds = systems.make_terms("bonds", a=i[:-1], b=i[1:], k=np.ones(n - 1), r0=np.full(n - 1, 1.5))
+    s.angles = systems.make_terms("angles", a=i[:-2], b=i[1:-1], c=i[2:], k=np.ones(n - 2), t0=np.ones(n - 2))
+    s.dihedrals = systems.make_terms("dihedrals", a=i[:-3], b=i[1:-2], c=i[2:-1], d=i[3:], k=np.ones(n - 3),
+                                     type=np.ones(n - 3, np.int32), mult=np.ones(n - 3))
+    o = O.Oracle(s)
+    rows = o.excl_rows()
+    pairs = {(a, b) for a in range(n) for b in rows[a]}
+    assert len(pairs) == (n - 1) + (n - 2) + (n - 3)
+    assert sorted(map(tuple, o.pairs14())) == [(k, k + 3) for k in range(n - 3)]
+
+
+def test_bonded_terms_at_reference_geometry(oracle_lib):
+    s, _ = systems.water_box(seed=4, nside=3, L=9.3)
+    e = O.Oracle(s).getE()
+    assert abs(e[5]) < 1e-20 and abs(e[6]) < 1e-18         # bonds at r0, angles at theta0
+    # planar trans dihedral: phi = pi, DCOS n=1 phi0=0 -> k(1 + cos(pi)) = 0
+    n = 4
+    s = systems.System(natom=n, box=[50.0] * 3, x=[-1.0, 0.0, 1.5, 2.5], y=[1.0, 0.0, 0.0, -1.0], z=[0.0] * 4)
+    s.dihedrals = systems.make_terms("dihedrals", a=[0], b=[1], c=[2], d=[3], k=[10.0], type=[1], mult=[1.0], phi0=[0.0])
+    assert abs(O.Oracle(s).getE()[8]) < 1e-12
+
+
+def test_rigid_moves_preserve_molecules(oracle_lib):
+    s, _ = systems.water_box(seed=8, nside=4, L=12.4)
+    o = O.Oracle(s)
+    o.update_list()
+    o.mc_run(400, 31, mode=1)
+    x = np.stack([s.x, s.y, s.z], 1).reshape(-1, 3, 3)
+    L = s.box[0]
+    d1 = x[:, 1] - x[:, 0]
+    d1 -= L * np.round(d1 / L)
+    d2 = x[:, 2] - x[:, 0]
+    d2 -= L * np.round(d2 / L)
+    r1, r2 = np.linalg.norm(d1, axis=1), np.linalg.norm(d2, axis=1)
+    intact = (np.abs(r1 - 0.9572) < 1e-6) & (np.abs(r2 - 0.9572) < 1e-6)
+    assert intact.mean() > 0.9        # molecules straddling the box edge may tear under ROT (reference quirk)
+    assert np.allclose(r1[intact], 0.9572, atol=1e-10) and np.allclose(r2[intact], 0.9572, atol=1e-10)
+
+
+def test_gradient_vs_central_differences(oracle_lib):
+    # 5 sites per side: no O-O pair sits exactly at L/2, where the minimum
+    # image distance has a kink
+    s, _ = systems.water_box(seed=6, nside=5, L=15.53)
+    o = O.Oracle(s)
+    o.update_list()
+    gx = o.grad_switch()[0]
+    h = 1e-5
+    for i in (0, 5, 17):
+        x0 = s.x[i]
+        s.x[i] = x0 + h
+        ep = o.getE()[3] + o.getE()[4]
+        s.x[i] = x0 - h
+        em = o.getE()[3] + o.getE()[4]
+        s.x[i] = x0
+        assert gx[i] == pytest.approx((ep - em) / (2 * h), rel=1e-5, abs=1e-3)
