@@ -160,6 +160,11 @@ int mcx_mc_run(mcx_ctx *h, const mcx_move_type *types, int ntypes, int nsteps, u
                int rng_mode, int list_every, mcx_mc_stats *stats, unsigned char *acc_out,
                double *de_out, int *mvtyp_out, int *mvinst_out);
 
+/* Philox stream id used by mcx_mc_run in MCX_MC_PHILOX mode (default 0).
+ * Replica r of a batch draws from stream r, so a handle set to stream r
+ * replays that replica's proposals.  Not in the reference. */
+int mcx_set_rng_stream(mcx_ctx *h, uint32_t stream);
+
 /* Adjoint of the switched nonbonded energy (list-based): dE/dx,dy,dz per
  * atom and dE/deps, dE/dsig, dE/dq per atom (iu, with eps/sig in the
  * AtomList convention).  Not in the reference. */
@@ -175,6 +180,87 @@ int mcx_device_coords(mcx_ctx *h, int which, void **ptr);
  * build, 3 fused bonded) on the handle's stream and returns the mean
  * device time per launch from HIP events. */
 int mcx_time_kernel(mcx_ctx *h, int which, int reps, double *ms_per_launch);
+
+/* ---------------------------------------------------------------------------
+ * Batched replicas.  Not in the reference, whose MC_metropolis::run
+ * (src/MC_metropolis.cpp:45-204) drives one chain: a batch runs one such
+ * chain per member handle, all replicas in one persistent kernel launch per
+ * sweep, with the same moves, draws and acceptance test.  Each replica's dE
+ * counts every pair whose minimum-image distance is inside the cutoff (no
+ * pair list), which equals the reference's list-based dE whenever no pair
+ * closes by more than the list skin between rebuilds.  Replica exchange
+ * swaps temperatures of neighbouring ladder slots on the device.
+ *
+ * Member handles must be fully set up (atoms, bonded terms, exclusions,
+ * coordinates, periodic box) on one device, with at most 32 atoms per move
+ * instance and at most 64 distinct (charge, eps, sig) atom types; they must
+ * outlive the batch.  The batch updates their device coordinates in place.
+ * After changing a member's set-up, call mcx_batch_reset.
+ */
+typedef struct mcx_batch mcx_batch;
+
+typedef struct mcx_batch_stats {
+    long long moves;               /* moves run by the call, all replicas */
+    long long accepted;            /* of which accepted */
+    long long exchange_attempts;   /* replica-exchange attempts of the call */
+    long long exchange_accepted;
+    long long sweeps;              /* sweeps since the last reset */
+    double seconds;                /* wall time of the call's device work */
+} mcx_batch_stats;
+
+#define MCX_MAX_MOVE_TYPES 8
+
+/* Creates a batch over nrep handles.  Replica r has ntypes[r] move types
+ * (1..MCX_MAX_MOVE_TYPES) as mcx_mc_run takes them; `types` holds them
+ * replica after replica (selections are per replica).
+ * Replica r starts in ladder slot r at its handle's temperature; the chains
+ * start as mcx_batch_reset(b, 0, MCX_MC_PHILOX, NULL) leaves them.  On
+ * failure the message is read with mcx_last_error(reps[0]). */
+int mcx_batch_create(mcx_ctx *const *reps, int nrep, const mcx_move_type *types, const int *ntypes,
+                     mcx_batch **out);
+void mcx_batch_destroy(mcx_batch *b);
+const char *mcx_batch_last_error(const mcx_batch *b);
+
+/* Restarts every chain from its handle's current coordinates: energies are
+ * recomputed, dmax and counters reset, replicas return to slot r, and the
+ * draws restart -- MCX_MC_REPLAY: std::mt19937(seeds[r]) (seeds NULL:
+ * seed + r) as mcx_mc_run draws; MCX_MC_PHILOX: key `seed`, stream r. */
+int mcx_batch_reset(mcx_batch *b, uint64_t seed, int rng_mode, const uint64_t *seeds);
+
+/* Temperatures (K) of the nrep ladder slots. */
+int mcx_batch_set_ladder(mcx_batch *b, const double *temps);
+
+/* Moves per sweep of each replica (default: its number of residues). */
+int mcx_batch_set_sweep(mcx_batch *b, const int *moves_per_sweep);
+
+/* Runs nsweeps sweeps of every replica; when exchange_every > 0, a
+ * replica-exchange round follows every exchange_every-th sweep (counted
+ * since the reset; rounds alternate between even and odd slot pairs).
+ * use_graph != 0 replays the sweeps and exchanges as a captured hipGraph. */
+int mcx_batch_run(mcx_batch *b, int nsweeps, int exchange_every, int use_graph, mcx_batch_stats *stats);
+
+/* Runs nmoves moves of every replica in one launch, with no exchange, and
+ * writes each replica's accepted flags and dE ([nrep][nmoves], either may
+ * be NULL). */
+int mcx_batch_moves(mcx_batch *b, int nmoves, unsigned char *acc_out, double *de_out);
+
+/* Per-replica state: running energy (iu), ladder slot, and per move type
+ * (MCX_MAX_MOVE_TYPES entries per replica, zero past its own types) trials,
+ * accepted moves and dmax.  Any pointer may be NULL. */
+int mcx_batch_state(mcx_batch *b, double *etot, int *slot_of_rep, long long *trials, long long *accepted,
+                    double *dmax);
+
+/* Replica-exchange attempts and acceptances of each neighbouring slot pair
+ * (k, k+1), nrep-1 entries each, since the last reset. */
+int mcx_batch_exchange_stats(mcx_batch *b, long long *attempts, long long *accepted);
+
+/* Launch shape: threads per replica workgroup (256, 512 or 1024) and the
+ * number of 64-candidate tiles each wavefront keeps in flight (1, 2, 4). */
+int mcx_batch_set_config(mcx_batch *b, int threads, int unroll);
+
+/* Autotuner: times one sweep of each launch shape on the batch as it is,
+ * restores the state after each trial, and keeps the fastest shape. */
+int mcx_batch_tune(mcx_batch *b, int *threads, int *unroll, double *ms_per_sweep);
 
 #ifdef __cplusplus
 }

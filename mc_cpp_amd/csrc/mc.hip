@@ -14,7 +14,7 @@
 // Philox4x32-10 stream), the test of apply_criterion (:175-204) and the
 // dmax adaptation (src/MC.cpp:165-173) all run in the same workgroup, so a
 // chunk of moves is one launch with no host round trip.
-#include "terms.h"
+#include "mcmove.h"
 
 #include <cfloat>
 
@@ -25,139 +25,6 @@ constexpr int kPairU = 4;          // list entries in flight per thread
 constexpr int kMaxRec = 96;        // term records of one move staged in LDS
 constexpr int kMCW = kMCT / 64;
 
-// ---------------------------------------------------------------------------
-// Random draws.  Wavefront 0 runs the serial part of every move with all 64
-// lanes in step; it keeps a batch of 32 draws in registers (lane l holds
-// draw l) and reads the next one with a wave-uniform shuffle.  A batch is
-// refilled in parallel: 64 MT19937 outputs (twist done by the wavefront in
-// four dependency-free phases) or 16 Philox4x32-10 blocks.
-
-constexpr int kMTN = 624;
-
-__device__ __forceinline__ uint32_t mt_temper(uint32_t y)
-{
-    y ^= (y >> 11);
-    y ^= (y << 7) & 0x9d2c5680u;
-    y ^= (y << 15) & 0xefc60000u;
-    y ^= (y >> 18);
-    return y;
-}
-
-__device__ __forceinline__ uint32_t mt_new(uint32_t cur, uint32_t nxt, uint32_t far)
-{
-    const uint32_t y = (cur & 0x80000000u) | (nxt & 0x7fffffffu);
-    uint32_t v = far ^ (y >> 1);
-    if (y & 1u) v ^= 0x9908b0dfu;
-    return v;
-}
-
-// MT19937 state regeneration by one wavefront.  New mt[i] depends on old
-// mt[i+1] and on mt[i+397] (old for i < 227, new for i >= 227), so the
-// ranges [0,227), [227,454), [454,623) and {623} are each free of
-// intra-phase dependencies; every phase reads all its inputs before writing.
-__device__ void mt_twist_wave(uint32_t *mt, int lane)
-{
-    const int lo[4] = {0, 227, 454, 623};
-    const int hi[4] = {227, 454, 623, 624};
-    for (int ph = 0; ph < 4; ph++) {
-        uint32_t v[4];
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int i = lo[ph] + r * 64 + lane;
-            if (i < hi[ph]) v[r] = mt_new(mt[i], mt[(i + 1) % kMTN], mt[(i + 397) % kMTN]);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int i = lo[ph] + r * 64 + lane;
-            if (i < hi[ph]) mt[i] = v[r];
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-}
-
-__device__ __forceinline__ double canonical2(uint32_t lo, uint32_t hi)
-{
-    // libstdc++ generate_canonical<double,53> over two 32-bit outputs
-    double sum = (double)lo;
-    sum += (double)hi * 4294967296.0;
-    double r = sum / 18446744073709551616.0;
-    if (r >= 1.0) r = 0.99999999999999989;
-    return r;
-}
-
-__device__ uint4 philox(uint4 c, uint2 k)
-{
-    for (int r = 0; r < 10; r++) {
-        const uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
-        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
-        c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
-        k.x += 0x9E3779B9u;
-        k.y += 0xBB67AE85u;
-    }
-    return c;
-}
-
-struct WaveDraws {
-    double v;          // this lane's draw of the current batch
-    int cur;           // next draw (uniform)
-    int mt_idx;        // MT19937 output index (uniform)
-    uint64_t ctr;      // Philox block counter (uniform)
-    bool replay;
-    uint2 key;
-    uint32_t chain;
-    uint32_t *mt;      // LDS state
-
-    __device__ void refill(int lane)
-    {
-        if (replay) {
-            const int p = mt_idx + lane;
-            uint32_t y = p < kMTN ? mt[p] : 0u;
-            if (mt_idx + 64 > kMTN) {
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                mt_twist_wave(mt, lane);
-                if (p >= kMTN) y = mt[p - kMTN];
-            }
-            // a batch ending exactly at 624 leaves the index at 624 (twist due
-            // before the next output); only a batch that crossed it wraps
-            mt_idx = mt_idx + 64 > kMTN ? mt_idx + 64 - kMTN : mt_idx + 64;
-            const uint32_t out = mt_temper(y);
-            const uint32_t lo = __shfl(out, (2 * lane) & 63, 64), hi = __shfl(out, (2 * lane + 1) & 63, 64);
-            v = canonical2(lo, hi);
-        } else {
-            uint4 w = make_uint4(0, 0, 0, 0);
-            if (lane < 16) w = philox(make_uint4((uint32_t)(ctr + lane), (uint32_t)((ctr + lane) >> 32), chain, 0x6d63u), key);
-            ctr += 16;
-            const int src = (lane >> 1) & 15;
-            const uint32_t x0 = __shfl(w.x, src, 64), x1 = __shfl(w.y, src, 64);
-            const uint32_t x2 = __shfl(w.z, src, 64), x3 = __shfl(w.w, src, 64);
-            const uint32_t a = (lane & 1) ? x2 : x0, b = (lane & 1) ? x3 : x1;
-            v = ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) / 9007199254740992.0;
-        }
-        cur = 0;
-    }
-    __device__ double next(int lane)
-    {
-        if (cur == 32) refill(lane);
-        const double u = __shfl(v, cur, 64);
-        cur++;
-        return u;
-    }
-};
-
-__device__ __forceinline__ double wsum(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
 // Shared state of one move.
 struct MoveLDS {
     int first, last, nm, total, acc, type, inst, pivot;
@@ -166,7 +33,7 @@ struct MoveLDS {
     double dx[kMaxMoveAtoms], dy[kMaxMoveAtoms], dz[kMaxMoveAtoms];   // unwrapped displacement of each moved atom
     int pref[kMaxMoveAtoms + 1];
     int4 ip[kMaxMoveAtoms];      // fixed-point mirror of the moved atoms (old positions)
-    float2 me[kMaxMoveAtoms];    // (eps, sig) of the moved atoms
+    double4 mq[kMaxMoveAtoms];   // (charge, eps, sig) of the moved atoms
     int mc[kMaxMoveAtoms];       // list-row length (or n) of the moved atoms
     double ox[kMaxMoveAtoms], oy[kMaxMoveAtoms], oz[kMaxMoveAtoms];   // old coordinates
     double ms[kMaxMoveAtoms];    // masses (centre-of-mass pivots)
@@ -183,11 +50,9 @@ struct MoveLDS {
 // (index b >= 0, new coordinates tx/ty/tz[b]) or fixed (b < 0).
 // Adds q_i-scaled elec_raw and e_i-scaled vdw_raw.
 __device__ void pair_delta(const DevSys &s, const MoveLDS &m, int i, int a, int j, int b, const int4 pj,
-                           const float2 ej, double &dec, double &dev)
+                           const double4 pq, double &dec, double &dev)
 {
     const int4 pi = m.ip[a];
-    const float qi = __int_as_float(pi.w), qj = __int_as_float(pj.w);
-    const float2 ei = m.me[a];
     double o[3];                                  // old displacement x_j - x_i
     if (s.pbtype != 0) {
         o[0] = (double)mcx_wdiff(pj.x, pi.x) * s.unit[0];
@@ -198,39 +63,12 @@ __device__ void pair_delta(const DevSys &s, const MoveLDS &m, int i, int a, int 
     }
     double g[3] = {-m.dx[a], -m.dy[a], -m.dz[a]};  // change of the displacement
     if (b >= 0) { g[0] += m.dx[b]; g[1] += m.dy[b]; g[2] += m.dz[b]; }
-    const double nx = o[0] + g[0], ny = o[1] + g[1], nz = o[2] + g[2];
-    const bool full = s.cutmode == 0;
-    const float sij = ei.y + ej.y;
-    bool consistent = true;
-    if (s.pbtype != 0)
-        consistent = fabs(nx) < 0.5 * s.L[0] && fabs(ny) < 0.5 * s.L[1] && fabs(nz) < 0.5 * s.L[2];
-    // the same energy function evaluated at the old and the new
-    // displacement: FP64 distance chain and Coulomb (the per-pair difference
-    // is then exact to ~1e-15 relative), FP32 LJ, FP64 switch
-    double dn[3] = {nx, ny, nz};
-    if (!consistent) {
-        // the minimum image changed between old and new displacement
-        double xjn, yjn, zjn;
-        if (b >= 0) { xjn = m.tx[b]; yjn = m.ty[b]; zjn = m.tz[b]; }
-        else { xjn = s.x[j]; yjn = s.y[j]; zjn = s.z[j]; }
-        dn[0] = xjn - m.tx[a]; dn[1] = yjn - m.ty[a]; dn[2] = zjn - m.tz[a];
-        mcx_mimg64(s, dn);
+    double c, v;
+    const double4 qa = m.mq[a];
+    if (mcx_pair_change(s, o[0], o[1], o[2], g[0], g[1], g[2], qa.x, qa.y, qa.z, pq.x, pq.y, pq.z, true, c, v)) {
+        dec += c;
+        dev += v;
     }
-    const double r2o = fma(o[2], o[2], fma(o[1], o[1], o[0] * o[0]));
-    const double r2n = fma(dn[2], dn[2], fma(dn[1], dn[1], dn[0] * dn[0]));
-    const bool ino = full || r2o <= s.ctoff2, inn = full || r2n <= s.ctoff2;
-    if (!ino && !inn) return;
-    float rio32, rin32;
-    const double rio = mcx_rsqrt(r2o, rio32), rin = mcx_rsqrt(r2n, rin32);
-    const float s2 = sij * sij;
-    const float to = s2 * rio32 * rio32, tn = s2 * rin32 * rin32;
-    const float t3o = to * to * to, t3n = tn * tn * tn;
-    const double So = ino ? (full ? 1.0 : mcx_switch(s, r2o)) : 0.0;
-    const double Sn = inn ? (full ? 1.0 : mcx_switch(s, r2n)) : 0.0;
-    const double dc = (double)qj * (Sn * rin - So * rio);
-    const double dv = (double)ej.x * (Sn * (double)fmaf(t3n, t3n, -t3n) - So * (double)fmaf(t3o, t3o, -t3o));
-    dec += (double)qi * dc;
-    dev += (double)ei.x * dv;
 }
 
 __device__ __forceinline__ bool excluded(const DevSys &s, int i, int j)
@@ -273,10 +111,10 @@ __device__ void move_delta(const DevSys &s, const DevTerms &t, const DevMoves &m
                 }
             }
             int4 pj[kPairU];
-            float2 ej[kPairU];
+            double4 pq[kPairU];
 #pragma unroll
             for (int u = 0; u < kPairU; u++) {
-                if (jv[u] >= 0) { pj[u] = s.ipos[jv[u]]; ej[u] = s.es[jv[u]]; }
+                if (jv[u] >= 0) { pj[u] = s.ipos[jv[u]]; pq[u] = s.qes[jv[u]]; }
             }
 #pragma unroll
             for (int u = 0; u < kPairU; u++) {
@@ -289,7 +127,7 @@ __device__ void move_delta(const DevSys &s, const DevTerms &t, const DevMoves &m
                     if (j < i) continue;     // pair of two moved atoms: counted once
                     b = j - m.first;
                 }
-                pair_delta(s, m, i, a, j, b, pj[u], ej[u], dec, dev);
+                pair_delta(s, m, i, a, j, b, pj[u], pq[u], dec, dev);
             }
         }
     } else if (m.ra1 >= 0) {
@@ -349,32 +187,6 @@ __device__ void move_delta(const DevSys &s, const DevTerms &t, const DevMoves &m
     __syncthreads();
 }
 
-// find_rot_mat, include/Move.hpp:77-118
-__device__ void rot_mat(double mat[3][3], double vec[3], double angle)
-{
-    const double angleRad = angle * MCX_PI / 180.0;
-    const double rn = 1.0 / sqrt(vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2]);
-    vec[0] *= rn; vec[1] *= rn; vec[2] *= rn;
-    double T[3][3], T2[3][3];
-    T[0][0] = T[1][1] = T[2][2] = 0.0;
-    T[2][1] = vec[0]; T[1][2] = -vec[0];
-    T[0][2] = vec[1]; T[2][0] = -vec[1];
-    T[1][0] = vec[2]; T[0][1] = -vec[2];
-    const double v1 = sin(angleRad), v2 = 1.0 - cos(angleRad);
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) { mat[i][j] = (i == j ? 1.0 : 0.0); mat[i][j] += T[i][j] * v1; T2[i][j] = 0.0; }
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++)
-            for (int k = 0; k < 3; k++) T2[i][j] += T[i][k] * T[k][j];
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) mat[i][j] += T2[i][j] * v2;
-}
-
-__device__ __forceinline__ double wrap1(const DevSys &s, double v, int k)
-{
-    return s.pbtype != 0 ? v - s.L[k] * mcx_ref_rint(s.rL[k] * v) : v;
-}
-
 // Phase B: every datum of the moved atoms, the pivot and the move's term
 // records in one round of parallel loads.
 __device__ void load_moved(const DevSys &s, const DevMoves &mv, MoveLDS &m)
@@ -384,7 +196,7 @@ __device__ void load_moved(const DevSys &s, const DevMoves &mv, MoveLDS &m)
         const int i = m.first + a;
         m.ox[a] = s.x[i]; m.oy[a] = s.y[i]; m.oz[a] = s.z[i];
         m.ip[a] = s.ipos[i];
-        m.me[a] = s.es[i];
+        m.mq[a] = s.qes[i];
         m.mc[a] = s.cutmode == 0 ? s.n : s.cnt[i];
         m.ms[a] = mv.mass ? mv.mass[i] : 1.0;
     } else if (a == kMaxMoveAtoms) {
@@ -458,19 +270,13 @@ __global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s, DevTerms t, DevMove
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const bool w0 = tid < 64;
-    // run state: [0,624) MT state, 624 MT index, 625/626 Philox counter,
-    // 627 cursor into the draw batch, 628.. the batch (32 doubles)
     WaveDraws dr;
     if (w0) {
         dr.replay = arg.rng_mode == 1;
         dr.mt = mt;
-        dr.mt_idx = (int)arg.mt_state[624];
-        dr.ctr = ((uint64_t)arg.mt_state[625] << 32) | arg.mt_state[626];
-        dr.cur = (int)arg.mt_state[627];
         dr.key = make_uint2((uint32_t)arg.seed, (uint32_t)(arg.seed >> 32));
-        dr.chain = 0;
-        const uint32_t *bv = arg.mt_state + 628;
-        dr.v = lane < 32 ? __hiloint2double((int)bv[2 * lane + 1], (int)bv[2 * lane]) : 0.0;
+        dr.chain = arg.stream;
+        dr.load(arg.mt_state, lane);
         if (lane < kMaxMoveTypes && lane < mv.ntypes) {
             dmax[lane] = arg.dmax[lane]; acc_tmp[lane] = arg.acc_tmp[lane];
             n_trial[lane] = arg.trials[lane]; n_acc[lane] = arg.accepted[lane];
@@ -485,36 +291,7 @@ __global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s, DevTerms t, DevMove
         const int st = arg.step0 + step + 1;
         // ---- phase A (wavefront 0, lanes in step): pick and draw the move
         // (src/MC_metropolis.cpp:85-116; src/MC.cpp:137-156)
-        int tp = 0;
-        if (w0) {
-            tp = (int)((double)mv.ntypes * dr.next(lane));
-            const int in = (int)((double)mv.ninst[tp] * dr.next(lane));
-            const int4 rec = mv.inst[mv.off[tp] + in];
-            const int2 rr = mv.inst_adj[mv.off[tp] + in];
-            double r[3];
-            do {
-                r[0] = dr.next(lane) * 2.0 + (-1.0);
-                r[1] = dr.next(lane) * 2.0 + (-1.0);
-                r[2] = dr.next(lane) * 2.0 + (-1.0);
-            } while ((r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) > 1.0);
-            const int type = mv.type[tp];
-            double R[3][3];
-            if (type == MCX_MV_TRN_DEV) {
-                r[0] = r[0] * dmax[tp]; r[1] = r[1] * dmax[tp]; r[2] = r[2] * dmax[tp];
-            } else {
-                const double rang = dmax[tp] * (dr.next(lane) * 2.0 + (-1.0));
-                rot_mat(R, r, rang);
-            }
-            if (lane == 0) {
-                m.type = type; m.inst = in; m.acc = tp;
-                m.first = rec.x; m.last = rec.y; m.pivot = rec.z;
-                m.ra0 = rr.x; m.ra1 = rr.y;
-                m.nm = rec.y - rec.x + 1;
-                m.r[0] = r[0]; m.r[1] = r[1]; m.r[2] = r[2];
-                if (type != MCX_MV_TRN_DEV)
-                    for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) m.R[a][b] = R[a][b];
-            }
-        }
+        if (w0) pick_move(mv, dr, dmax, m, lane);
         __syncthreads();
         load_moved(s, mv, m);
         make_trial(s, m);
@@ -523,14 +300,9 @@ __global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s, DevTerms t, DevMove
         // src/MC_metropolis.cpp:175-204), bookkeeping, commit of the moved
         // atoms; the next phase-A barrier publishes the commit
         if (w0) {
-            tp = m.acc;
+            const int tp = m.acc;
             const double de = m.de;
-            int acc;
-            if (de <= DBL_EPSILON) acc = 1;
-            else {
-                const double alpha = dr.next(lane);
-                acc = alpha < exp(-arg.beta * de);
-            }
+            const int acc = metropolis(dr, arg.beta, de, lane);
             if (lane == 0) {
                 n_trial[tp]++;
                 if (acc) { etot += de; n_acc[tp]++; }
@@ -539,15 +311,7 @@ __global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s, DevTerms t, DevMove
                 if (arg.de_out) arg.de_out[o] = de;
                 if (arg.mvtyp_out) arg.mvtyp_out[o] = tp;
                 if (arg.mvinst_out) arg.mvinst_out[o] = m.inst;
-                if (acc) acc_tmp[tp]++;
-                for (int k = 0; k < mv.ntypes; k++) {
-                    if (mv.each[k] != 0 && (st % mv.each[k]) == 0) {
-                        const double ratio = (double)acc_tmp[k] / (double)mv.each[k];
-                        if (ratio > mv.target[k] / 100) dmax[k] *= 1.10;
-                        else dmax[k] *= 0.9;
-                        acc_tmp[k] = 0;
-                    }
-                }
+                adapt_dmax(mv, st, tp, acc, dmax, acc_tmp);
             }
             if (acc && lane < m.nm) {
                 const int i = m.first + lane;
@@ -562,17 +326,8 @@ __global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s, DevTerms t, DevMove
             arg.dmax[lane] = dmax[lane]; arg.acc_tmp[lane] = acc_tmp[lane];
             arg.trials[lane] = n_trial[lane]; arg.accepted[lane] = n_acc[lane];
         }
-        if (lane == 0) {
-            arg.etot[0] = etot;
-            arg.mt_state[624] = (uint32_t)dr.mt_idx;
-            arg.mt_state[625] = (uint32_t)(dr.ctr >> 32);
-            arg.mt_state[626] = (uint32_t)dr.ctr;
-            arg.mt_state[627] = (uint32_t)dr.cur;
-        }
-        if (lane < 32) {
-            arg.mt_state[628 + 2 * lane] = (uint32_t)__double2loint(dr.v);
-            arg.mt_state[628 + 2 * lane + 1] = (uint32_t)__double2hiint(dr.v);
-        }
+        if (lane == 0) arg.etot[0] = etot;
+        dr.save(arg.mt_state, lane);
     }
     if (arg.rng_mode == 1)
         for (int k = tid; k < kMTN; k += kMCT) arg.mt_state[k] = mt[k];

@@ -1,0 +1,315 @@
+// mcmove.h -- device code shared by the single-chain Metropolis kernel and
+// the batched replica kernel: the random draws (std::mt19937 replay or
+// Philox4x32-10), the rigid-body move of include/Move.hpp and the change of
+// one pair's nonbonded energy under a move.
+#pragma once
+
+#include "terms.h"
+
+#include <cfloat>
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Random draws.  Wavefront 0 of a workgroup runs the serial part of every
+// move with all 64 lanes in step; it keeps a batch of 32 draws in registers
+// (lane l holds draw l) and reads the next one with a wave-uniform shuffle.
+// A batch is refilled in parallel: 64 MT19937 outputs (twist done by the
+// wavefront in four dependency-free phases) or 16 Philox4x32-10 blocks.
+
+constexpr int kMTN = 624;
+
+__device__ __forceinline__ uint32_t mt_temper(uint32_t y)
+{
+    y ^= (y >> 11);
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= (y >> 18);
+    return y;
+}
+
+__device__ __forceinline__ uint32_t mt_new(uint32_t cur, uint32_t nxt, uint32_t far)
+{
+    const uint32_t y = (cur & 0x80000000u) | (nxt & 0x7fffffffu);
+    uint32_t v = far ^ (y >> 1);
+    if (y & 1u) v ^= 0x9908b0dfu;
+    return v;
+}
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// MT19937 state regeneration by one wavefront.  New mt[i] depends on old
+// mt[i+1] and on mt[i+397] (old for i < 227, new for i >= 227), so the
+// ranges [0,227), [227,454), [454,623) and {623} are each free of
+// intra-phase dependencies; every phase reads all its inputs before writing.
+__device__ void mt_twist_wave(uint32_t *mt, int lane)
+{
+    const int lo[4] = {0, 227, 454, 623};
+    const int hi[4] = {227, 454, 623, 624};
+    for (int ph = 0; ph < 4; ph++) {
+        uint32_t v[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int i = lo[ph] + r * 64 + lane;
+            if (i < hi[ph]) v[r] = mt_new(mt[i], mt[(i + 1) % kMTN], mt[(i + 397) % kMTN]);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int i = lo[ph] + r * 64 + lane;
+            if (i < hi[ph]) mt[i] = v[r];
+        }
+        wave_sync();
+    }
+}
+
+__device__ __forceinline__ double canonical2(uint32_t lo, uint32_t hi)
+{
+    // libstdc++ generate_canonical<double,53> over two 32-bit outputs
+    double sum = (double)lo;
+    sum += (double)hi * 4294967296.0;
+    double r = sum / 18446744073709551616.0;
+    if (r >= 1.0) r = 0.99999999999999989;
+    return r;
+}
+
+__device__ uint4 philox(uint4 c, uint2 k)
+{
+    for (int r = 0; r < 10; r++) {
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+        k.x += 0x9E3779B9u;
+        k.y += 0xBB67AE85u;
+    }
+    return c;
+}
+
+// 53-bit uniform in [0,1) from two Philox words
+__device__ __forceinline__ double philox_u53(uint32_t a, uint32_t b)
+{
+    return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) / 9007199254740992.0;
+}
+
+// The saved draw state has the kRngWords layout of mcx_internal.h.
+struct WaveDraws {
+    double v;          // this lane's draw of the current batch
+    int cur;           // next draw (uniform)
+    int mt_idx;        // MT19937 output index (uniform)
+    uint64_t ctr;      // Philox block counter (uniform)
+    bool replay;
+    uint2 key;
+    uint32_t chain;    // Philox stream
+    uint32_t *mt;      // LDS state
+
+    __device__ void load(const uint32_t *st, int lane)
+    {
+        mt_idx = (int)st[624];
+        ctr = ((uint64_t)st[625] << 32) | st[626];
+        cur = (int)st[627];
+        v = lane < 32 ? __hiloint2double((int)st[628 + 2 * lane + 1], (int)st[628 + 2 * lane]) : 0.0;
+    }
+    __device__ void save(uint32_t *st, int lane) const
+    {
+        if (lane == 0) {
+            st[624] = (uint32_t)mt_idx;
+            st[625] = (uint32_t)(ctr >> 32);
+            st[626] = (uint32_t)ctr;
+            st[627] = (uint32_t)cur;
+        }
+        if (lane < 32) {
+            st[628 + 2 * lane] = (uint32_t)__double2loint(v);
+            st[628 + 2 * lane + 1] = (uint32_t)__double2hiint(v);
+        }
+    }
+    __device__ void refill(int lane)
+    {
+        if (replay) {
+            const int p = mt_idx + lane;
+            uint32_t y = p < kMTN ? mt[p] : 0u;
+            if (mt_idx + 64 > kMTN) {
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                mt_twist_wave(mt, lane);
+                if (p >= kMTN) y = mt[p - kMTN];
+            }
+            // a batch ending exactly at 624 leaves the index at 624 (twist due
+            // before the next output); only a batch that crossed it wraps
+            mt_idx = mt_idx + 64 > kMTN ? mt_idx + 64 - kMTN : mt_idx + 64;
+            const uint32_t out = mt_temper(y);
+            const uint32_t lo = __shfl(out, (2 * lane) & 63, 64), hi = __shfl(out, (2 * lane + 1) & 63, 64);
+            v = canonical2(lo, hi);
+        } else {
+            uint4 w = make_uint4(0, 0, 0, 0);
+            if (lane < 16) w = philox(make_uint4((uint32_t)(ctr + lane), (uint32_t)((ctr + lane) >> 32), chain, 0x6d63u), key);
+            ctr += 16;
+            const int src = (lane >> 1) & 15;
+            const uint32_t x0 = __shfl(w.x, src, 64), x1 = __shfl(w.y, src, 64);
+            const uint32_t x2 = __shfl(w.z, src, 64), x3 = __shfl(w.w, src, 64);
+            const uint32_t a = (lane & 1) ? x2 : x0, b = (lane & 1) ? x3 : x1;
+            v = philox_u53(a, b);
+        }
+        cur = 0;
+    }
+    __device__ double next(int lane)
+    {
+        if (cur == 32) refill(lane);
+        const double u = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), cur),
+                                          __builtin_amdgcn_readlane(__double2loint(v), cur));
+        cur++;
+        return u;
+    }
+};
+
+__device__ __forceinline__ double wsum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wmax(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// find_rot_mat, include/Move.hpp:77-118
+__device__ void rot_mat(double mat[3][3], double vec[3], double angle)
+{
+    const double angleRad = angle * MCX_PI / 180.0;
+    const double rn = 1.0 / sqrt(vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2]);
+    vec[0] *= rn; vec[1] *= rn; vec[2] *= rn;
+    double T[3][3], T2[3][3];
+    T[0][0] = T[1][1] = T[2][2] = 0.0;
+    T[2][1] = vec[0]; T[1][2] = -vec[0];
+    T[0][2] = vec[1]; T[2][0] = -vec[1];
+    T[1][0] = vec[2]; T[0][1] = -vec[2];
+    const double v1 = sin(angleRad), v2 = 1.0 - cos(angleRad);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) { mat[i][j] = (i == j ? 1.0 : 0.0); mat[i][j] += T[i][j] * v1; T2[i][j] = 0.0; }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++) T2[i][j] += T[i][k] * T[k][j];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) mat[i][j] += T2[i][j] * v2;
+}
+
+__device__ __forceinline__ double wrap1(const DevSys &s, double v, int k)
+{
+    return s.pbtype != 0 ? v - s.L[k] * mcx_ref_rint(s.rL[k] * v) : v;
+}
+
+// Phase A of a move (src/MC_metropolis.cpp:85-116; src/MC.cpp:137-156), run
+// by wavefront 0 with its lanes in step: move type, instance, the rndSphere
+// direction (rejection in the unit ball) and, for rotations, the angle and
+// the rotation matrix.  Lane 0 publishes the move in m.
+template <class M>
+__device__ void pick_move(const DevMoves &mv, WaveDraws &dr, const double *dmax, M &m, int lane)
+{
+    const int tp = (int)((double)mv.ntypes * dr.next(lane));
+    const int in = (int)((double)mv.ninst[tp] * dr.next(lane));
+    const int4 rec = mv.inst[mv.off[tp] + in];
+    const int2 rr = mv.inst_adj[mv.off[tp] + in];
+    double r[3];
+    do {
+        r[0] = dr.next(lane) * 2.0 + (-1.0);
+        r[1] = dr.next(lane) * 2.0 + (-1.0);
+        r[2] = dr.next(lane) * 2.0 + (-1.0);
+    } while ((r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) > 1.0);
+    const int type = mv.type[tp];
+    double R[3][3];
+    if (type == MCX_MV_TRN_DEV) {
+        r[0] = r[0] * dmax[tp]; r[1] = r[1] * dmax[tp]; r[2] = r[2] * dmax[tp];
+    } else {
+        const double rang = dmax[tp] * (dr.next(lane) * 2.0 + (-1.0));
+        rot_mat(R, r, rang);
+    }
+    if (lane == 0) {
+        m.type = type; m.inst = in; m.acc = tp;
+        m.first = rec.x; m.last = rec.y; m.pivot = rec.z;
+        m.ra0 = rr.x; m.ra1 = rr.y;
+        m.nm = rec.y - rec.x + 1;
+        m.r[0] = r[0]; m.r[1] = r[1]; m.r[2] = r[2];
+        if (type != MCX_MV_TRN_DEV)
+            for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) m.R[a][b] = R[a][b];
+    }
+}
+
+// MC_metropolis::apply_criterion (src/MC_metropolis.cpp:175-204): a draw is
+// consumed only when dE exceeds DBL_EPSILON.
+__device__ __forceinline__ int metropolis(WaveDraws &dr, double beta, double de, int lane)
+{
+    if (de <= DBL_EPSILON) return 1;
+    const double alpha = dr.next(lane);
+    return alpha < exp(-beta * de);
+}
+
+// Acceptance bookkeeping and the dmax update of MC::adjust_dmax
+// (src/MC.cpp:165-173) after step st (1-based).
+__device__ __forceinline__ void adapt_dmax(const DevMoves &mv, long long st, int tp, int acc, double *dmax,
+                                           int *acc_tmp)
+{
+    if (acc) acc_tmp[tp]++;
+    for (int k = 0; k < mv.ntypes; k++) {
+        if (mv.each[k] != 0 && (st % mv.each[k]) == 0) {
+            const double ratio = (double)acc_tmp[k] / (double)mv.each[k];
+            if (ratio > mv.target[k] / 100) dmax[k] *= 1.10;
+            else dmax[k] *= 0.9;
+            acc_tmp[k] = 0;
+        }
+    }
+}
+
+// Change of one pair's nonbonded sums when atom i and possibly j move.
+// (ox, oy, oz) is the old minimum-image displacement x_j - x_i and g its
+// change.  The new displacement is o + g, re-imaged when it leaves the half
+// box on some axis (tested only when `check` is set: a caller that knows
+// |o + g| < L/2 on every axis skips it), so old and new refer to the same
+// stored positions.  The pair energy (charges, eps, sig as given, FP64) is
+// evaluated at both displacements in FP64 and differenced: a move's dE is a
+// small difference of pair terms up to 1e4 times larger, so the FP32
+// rounding of a parameter alone (~6e-8) would exceed the 1e-5 bar on dE
+// (on gfx950, FP64 FMA issues at the non-packed FP32 rate).  1/r comes
+// from the FP32 estimate and one FP64 Newton step (~1e-14).  Returns false
+// (and adds nothing) when neither distance is inside the cutoff; otherwise
+// c = q_i*dC and v = eps_i*dV, the raw sums before the 332.0716*418.4 and
+// 4 factors of src/FField_MDBAS.cpp:420-421.
+__device__ __forceinline__ bool mcx_pair_change(const DevSys &s, double ox, double oy, double oz, double gx, double gy,
+                                                double gz, double qi, double ei, double si, double qj, double ej,
+                                                double sj, bool check, double &c, double &v)
+{
+    double dn[3] = {ox + gx, oy + gy, oz + gz};
+    const bool full = s.cutmode == 0;
+    if (check && s.pbtype != 0 &&
+        !(fabs(dn[0]) < 0.5 * s.L[0] && fabs(dn[1]) < 0.5 * s.L[1] && fabs(dn[2]) < 0.5 * s.L[2]))
+        mcx_mimg64(s, dn);
+    const double r2o = fma(oz, oz, fma(oy, oy, ox * ox));
+    const double r2n = fma(dn[2], dn[2], fma(dn[1], dn[1], dn[0] * dn[0]));
+    const bool ino = full || r2o <= s.ctoff2, inn = full || r2n <= s.ctoff2;
+    if (!ino && !inn) return false;
+    float f32;
+    const double rio = mcx_rsqrt(r2o, f32), rin = mcx_rsqrt(r2n, f32);
+    const double So = ino ? (full ? 1.0 : mcx_switch(s, r2o)) : 0.0;
+    const double Sn = inn ? (full ? 1.0 : mcx_switch(s, r2n)) : 0.0;
+    const double sij = si + sj;
+    const double s2 = sij * sij;
+    const double to = s2 * (rio * rio), tn = s2 * (rin * rin);
+    const double t3o = to * to * to, t3n = tn * tn * tn;
+    const double dc = qj * (Sn * rin - So * rio);
+    const double dv = ej * (Sn * fma(t3n, t3n, -t3n) - So * fma(t3o, t3o, -t3o));
+    c = qi * dc;
+    v = ei * dv;
+    return true;
+}
+
+}  // namespace

@@ -1,8 +1,7 @@
 // mcx_api.cpp -- host side of the gfx950 energy backend: the C ABI of
 // include/mcx.h, device buffer management, the exclusion-list builder and
 // the move-instance tables (both host work in the reference too).
-#include "../../include/mcx.h"
-#include "mcx_internal.h"
+#include "mcx_ctx.h"
 
 #include <algorithm>
 #include <chrono>
@@ -12,155 +11,7 @@
 #include <map>
 #include <memory>
 
-namespace {
-
-template <typename T>
-struct DBuf {
-    T *p = nullptr;
-    size_t n = 0;
-    void alloc(size_t count)
-    {
-        if (count <= n && p) return;
-        if (p) HIPCHK(hipFree(p));
-        p = nullptr;
-        n = 0;
-        if (count == 0) count = 1;
-        HIPCHK(hipMalloc((void **)&p, count * sizeof(T)));
-        n = count;
-    }
-    void upload(const T *h, size_t count, hipStream_t st)
-    {
-        alloc(count);
-        if (count) HIPCHK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, st));
-    }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-};
-
-}  // namespace
-
-struct mcx_ctx {
-    int n = 0, device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    // parameters
-    int pbtype = MCX_PBC_CUBIC;
-    double box[3] = {0, 0, 0};
-    int cutmode = MCX_CUT_SWITCH;
-    double ctoff = 12.0, cton = 10.0, dcut = 2.0, temp = 300.0;
-    bool have_atoms = false, have_coords = false, have_excl = false, list_valid = false;
-    std::vector<double> q, eps, sig, eps14, sig14, mass;
-    std::vector<int> frozen, resid;
-    HostTerms terms;    // kinds 0..4 from the caller, 5/6 from the exclusion builder
-    std::vector<int> excl_pair, excl_off, excl_idx;   // visible rows
-    std::vector<int> pairs14;
-    // device buffers
-    DBuf<double> x, y, z, q64, eps64, sig64, eps1464, sig1464, mass64;
-    DBuf<int4> ipos;
-    DBuf<float2> es, esk, es14;
-    std::vector<double> hx, hy, hz;   // host copy of the last uploaded coordinates
-    DBuf<int> frozen_d, excl_off_d, excl_idx_d, xfull_off_d, xfull_idx_d;
-    DBuf<double4> cpos;
-    DBuf<int> nbr, cnt, half, overflow;
-    int cap = 0;
-    DBuf<int> cell_of, cell_start, cell_count, cell_atoms, cursor;
-    CellGrid grid{};
-    DBuf<int2> tiles;
-    int ntiles = 0;
-    DBuf<double> partial, result;
-    DBuf<int4> t_idx[kNumKinds];
-    DBuf<int> t_type[kNumKinds];
-    DBuf<double4> t_par[kNumKinds];
-    DBuf<int> adj_off, adj_kk, adj_type;
-    DBuf<int4> adj_ids;
-    DBuf<double4> adj_par;
-    std::vector<int> adj_off_h, adj_kk_h;
-    DBuf<int4> mv_rid;
-    DBuf<int> mv_rkk, mv_rty;
-    DBuf<double4> mv_rpar;
-    DBuf<int2> mv_inst_adj;
-    DBuf<int4> mv_inst;
-    DBuf<double> newxyz, scalar;
-    long long last_npairs = 0;
-
-    DevSys sys() const
-    {
-        DevSys s{};
-        s.n = n;
-        s.pbtype = pbtype;
-        for (int k = 0; k < 3; k++) {
-            s.L[k] = box[k];
-            s.rL[k] = pbtype == MCX_PBC_NONE ? 0.0 : 1.0 / box[k];
-            s.Lf[k] = (float)s.L[k];
-            s.rLf[k] = (float)s.rL[k];
-        }
-        s.ctoff2 = ctoff * ctoff;
-        s.cton2 = cton * cton;
-        const double dd = s.ctoff2 - s.cton2;
-        s.switch2 = 1. / (dd * dd * dd);
-        double cn = ctoff + dcut;
-        s.cutnb2 = cn * cn;
-        s.cutmode = cutmode;
-        s.x = x.p; s.y = y.p; s.z = z.p;
-        s.ipos = ipos.p;
-        s.es = es.p; s.esk = esk.p; s.es14 = es14.p;
-        for (int k = 0; k < 3; k++) s.unit[k] = pbtype == MCX_PBC_NONE ? 1.0 : box[k] / 4294967296.0;
-        s.iunit = pbtype == MCX_PBC_CUBIC ? 4294967296.0 / box[0] : 1.0;
-        s.q64 = q64.p; s.eps64 = eps64.p; s.sig64 = sig64.p; s.eps1464 = eps1464.p; s.sig1464 = sig1464.p;
-        s.frozen = frozen_d.p;
-        s.excl_off = excl_off_d.p; s.excl_idx = excl_idx_d.p;
-        s.xfull_off = xfull_off_d.p; s.xfull_idx = xfull_idx_d.p;
-        s.cap = cap;
-        s.nbr = nbr.p; s.cnt = cnt.p; s.half = half.p; s.overflow = overflow.p;
-        return s;
-    }
-
-    DevTerms dterms() const
-    {
-        DevTerms t{};
-        t.start[0] = 0;
-        for (int k = 0; k < kNumKinds; k++) {
-            t.nterm[k] = (int)terms.idx[k].size();
-            t.start[k + 1] = t.start[k] + t.nterm[k];
-            t.idx[k] = t_idx[k].p;
-            t.type[k] = t_type[k].p;
-            t.par[k] = t_par[k].p;
-        }
-        // the excluded-pair correction applies to FULL mode only
-        if (cutmode != MCX_CUT_FULL) {
-            t.nterm[kExclPair] = 0;
-            t.start[kExclPair + 1] = t.start[kExclPair];
-        }
-        return t;
-    }
-
-    ~mcx_ctx()
-    {
-        DBuf<double> *d[] = {&x, &y, &z, &q64, &eps64, &sig64, &eps1464, &sig1464, &mass64, &partial, &result,
-                             &newxyz, &scalar};
-        for (auto *b : d) b->release();
-        ipos.release(); es.release(); esk.release(); es14.release();
-        cpos.release();
-        adj_ids.release();
-        adj_par.release();
-        adj_type.release();
-        mv_inst_adj.release();
-        mv_rid.release(); mv_rkk.release(); mv_rty.release(); mv_rpar.release();
-        mv_inst.release();
-        DBuf<int> *ib[] = {&frozen_d, &excl_off_d, &excl_idx_d, &xfull_off_d, &xfull_idx_d, &nbr, &cnt, &half, &overflow, &cell_of,
-                           &cell_start, &cell_count, &cell_atoms, &cursor, &adj_off, &adj_kk};
-        for (auto *b : ib) b->release();
-        tiles.release();
-        for (int k = 0; k < kNumKinds; k++) { t_idx[k].release(); t_type[k].release(); t_par[k].release(); }
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
-
-namespace {
+namespace mcxi {
 
 int fail(mcx_ctx *h, int code, const std::string &msg)
 {
@@ -342,6 +193,8 @@ void upload_terms(mcx_ctx *h)
     }
     h->xfull_off_d.upload(xo.data(), xo.size(), h->stream);
     h->xfull_idx_d.upload(xi.data(), xi.size(), h->stream);
+    h->xfull_off_h = xo;
+    h->xfull_idx_h = xi;
 }
 
 void setup_grid(mcx_ctx *h)
@@ -568,7 +421,7 @@ void residue_range(const std::vector<int> &resLast, int atomidx, int &a1, int &a
     throw mcx_error(MCX_ERR_ARG, "residue bounds not found");
 }
 
-DevMoves build_moves(mcx_ctx *h, const mcx_move_type *types, int ntypes)
+DevMoves build_moves(mcx_ctx *h, const mcx_move_type *types, int ntypes, MoveBufs &buf, int max_atoms)
 {
     if (ntypes < 1 || ntypes > kMaxMoveTypes) throw mcx_error(MCX_ERR_ARG, "1..8 move types supported");
     DevMoves mv{};
@@ -591,28 +444,57 @@ DevMoves build_moves(mcx_ctx *h, const mcx_move_type *types, int ntypes)
                 if (m.sele && !m.sele[i]) continue;
                 int a1 = i, a2 = i;
                 if (m.mode == MCX_MODE_RESIDUE) residue_range(resLast, i, a1, a2);
-                if (a2 - a1 + 1 > kMaxMoveAtoms)
-                    throw mcx_error(MCX_ERR_UNSUPPORTED, "move instance larger than 64 atoms");
+                if (a2 - a1 + 1 > max_atoms)
+                    throw mcx_error(MCX_ERR_UNSUPPORTED, "move instance larger than " + std::to_string(max_atoms) +
+                                                             " atoms");
                 F.push_back(a1); L.push_back(a2); P.push_back(m.type == MCX_MV_ROT ? i : -1);
             }
         }
         mv.ninst[t] = (int)F.size() - mv.off[t];
         if (mv.ninst[t] == 0) throw mcx_error(MCX_ERR_ARG, "empty move selection");
-        if (m.mode == MCX_MODE_ALL && h->n > kMaxMoveAtoms)
-            throw mcx_error(MCX_ERR_UNSUPPORTED, "ALL-mode moves of more than 64 atoms");
+        if (m.mode == MCX_MODE_ALL && h->n > max_atoms)
+            throw mcx_error(MCX_ERR_UNSUPPORTED, "ALL-mode moves of more than " + std::to_string(max_atoms) +
+                                                     " atoms");
     }
     // every bonded term / 1-4 pair touching an instance's atom range, once,
     // as self-contained records (atoms, kind|index, type, parameters);
     // instances with the same range share one record list
     std::vector<int4> rec(F.size());
-    std::vector<int2> radj(F.size());
-    std::map<std::pair<int, int>, int2> seen;
+    std::vector<int2> radj(F.size()), rpairs(F.size());
+    std::map<std::pair<int, int>, int2> seen, seen_p;
+    std::vector<int2> pent;
     std::vector<int4> rid;
     std::vector<int> rkk, rty;
     std::vector<double4> rpar;
     for (size_t k = 0; k < F.size(); k++) {
         rec[k] = make_int4(F[k], L[k], P[k], 0);
         const auto key = std::make_pair(F[k], L[k]);
+        // pair corrections of a list-free dE over all atoms: excluded
+        // partners outside the range (kind 0, to subtract) and the
+        // non-excluded pairs inside it (kind 1, to add); pairs of two frozen
+        // atoms are in neither
+        auto itp = seen_p.find(key);
+        if (itp != seen_p.end()) {
+            rpairs[k] = itp->second;
+        } else {
+            const int begin = (int)pent.size();
+            for (int a = F[k]; a <= L[k]; a++) {
+                const int x0 = h->xfull_off_h[a], x1 = h->xfull_off_h[a + 1];
+                for (int e = x0; e < x1; e++) {
+                    const int j = h->xfull_idx_h[e];
+                    if (j >= F[k] && j <= L[k]) continue;
+                    if (h->frozen[a] && h->frozen[j]) continue;
+                    pent.push_back(make_int2(a - F[k], j));
+                }
+                for (int b = a + 1; b <= L[k]; b++) {
+                    if (std::binary_search(h->xfull_idx_h.begin() + x0, h->xfull_idx_h.begin() + x1, b)) continue;
+                    if (h->frozen[a] && h->frozen[b]) continue;
+                    pent.push_back(make_int2((a - F[k]) | (1 << 16), b - F[k]));
+                }
+            }
+            rpairs[k] = make_int2(begin, (int)pent.size());
+            seen_p[key] = rpairs[k];
+        }
         auto it = seen.find(key);
         if (it != seen.end()) { radj[k] = it->second; continue; }
         const int begin = (int)rid.size();
@@ -631,18 +513,22 @@ DevMoves build_moves(mcx_ctx *h, const mcx_move_type *types, int ntypes)
         radj[k] = make_int2(begin, (int)rid.size());
         seen[key] = radj[k];
     }
-    h->mv_rid.upload(rid.data(), rid.size(), h->stream);
-    h->mv_rkk.upload(rkk.data(), rkk.size(), h->stream);
-    h->mv_rty.upload(rty.data(), rty.size(), h->stream);
-    h->mv_rpar.upload(rpar.data(), rpar.size(), h->stream);
-    h->mv_inst.upload(rec.data(), rec.size(), h->stream);
-    h->mv_inst_adj.upload(radj.data(), radj.size(), h->stream);
-    mv.inst = h->mv_inst.p;
-    mv.inst_adj = h->mv_inst_adj.p;
-    mv.rec_ids = h->mv_rid.p;
-    mv.rec_kk = h->mv_rkk.p;
-    mv.rec_type = h->mv_rty.p;
-    mv.rec_par = h->mv_rpar.p;
+    buf.rid.upload(rid.data(), rid.size(), h->stream);
+    buf.rkk.upload(rkk.data(), rkk.size(), h->stream);
+    buf.rty.upload(rty.data(), rty.size(), h->stream);
+    buf.rpar.upload(rpar.data(), rpar.size(), h->stream);
+    buf.inst.upload(rec.data(), rec.size(), h->stream);
+    buf.inst_adj.upload(radj.data(), radj.size(), h->stream);
+    buf.inst_pairs.upload(rpairs.data(), rpairs.size(), h->stream);
+    buf.pair_ent.upload(pent.data(), pent.size(), h->stream);
+    mv.inst_pairs = buf.inst_pairs.p;
+    mv.pair_ent = buf.pair_ent.p;
+    mv.inst = buf.inst.p;
+    mv.inst_adj = buf.inst_adj.p;
+    mv.rec_ids = buf.rid.p;
+    mv.rec_kk = buf.rkk.p;
+    mv.rec_type = buf.rty.p;
+    mv.rec_par = buf.rpar.p;
     mv.mass = h->mass64.p;
     mv.adj_off = h->adj_off.p;
     mv.adj_ids = h->adj_ids.p;
@@ -671,7 +557,9 @@ void mt_seed(uint32_t *st, uint64_t seed)
     st[624] = 624;
 }
 
-}  // namespace
+}  // namespace mcxi
+
+using namespace mcxi;
 
 // ---------------------------------------------------------------------------
 // C ABI
@@ -757,6 +645,9 @@ int mcx_set_atoms(mcx_ctx *h, const double *charge, const double *eps, const dou
         }
         h->es.upload(e.data(), n, h->stream);
         h->es14.upload(e14.data(), n, h->stream);
+        std::vector<double4> qes(n);
+        for (int i = 0; i < n; i++) qes[i] = make_double4(h->q[i], h->eps[i], h->sig[i], 0.0);
+        h->qes.upload(qes.data(), n, h->stream);
         h->q64.upload(h->q.data(), n, h->stream);
         h->eps64.upload(h->eps.data(), n, h->stream);
         h->sig64.upload(h->sig.data(), n, h->stream);
@@ -962,7 +853,7 @@ int mcx_mc_run(mcx_ctx *h, const mcx_move_type *types, int ntypes, int nsteps, u
         if (nsteps < 0) throw mcx_error(MCX_ERR_ARG, "nsteps < 0");
         if (!h->have_excl || !h->have_coords || !h->have_atoms) throw mcx_error(MCX_ERR_STATE, "system not set up");
         if (h->cutmode != MCX_CUT_FULL && !h->list_valid) build_list(h);
-        DevMoves mv = build_moves(h, types, ntypes);
+        DevMoves mv = build_moves(h, types, ntypes, h->mvb, kMaxMoveAtoms);
         // device-side run state
         DBuf<double> etot, dmax;
         DBuf<uint32_t> mts;
@@ -1001,6 +892,7 @@ int mcx_mc_run(mcx_ctx *h, const mcx_move_type *types, int ntypes, int nsteps, u
         MCRunArgs a{};
         a.rng_mode = rng_mode == MCX_MC_REPLAY ? 1 : 0;
         a.seed = seed;
+        a.stream = h->rng_stream;
         a.beta = 1.0 / ((MCX_RBOLTZUI / MCX_KCALTOIU) * h->temp);
         a.etot = etot.p; a.mt_state = mts.p; a.dmax = dmax.p; a.trials = trials.p; a.accepted = accepted.p;
         a.acc_tmp = acc_tmp.p;
@@ -1043,6 +935,11 @@ int mcx_mc_run(mcx_ctx *h, const mcx_move_type *types, int ntypes, int nsteps, u
         if (mvtyp_out && nsteps) HIPCHK(hipMemcpy(mvtyp_out, typ_d.p, nsteps * sizeof(int), hipMemcpyDeviceToHost));
         if (mvinst_out && nsteps) HIPCHK(hipMemcpy(mvinst_out, inst_d.p, nsteps * sizeof(int), hipMemcpyDeviceToHost));
     });
+}
+
+int mcx_set_rng_stream(mcx_ctx *h, uint32_t stream)
+{
+    return guarded(h, [&] { h->rng_stream = stream; });
 }
 
 int mcx_nonbonded_grad(mcx_ctx *h, double *gx, double *gy, double *gz, double *geps, double *gsig, double *gq)

@@ -1,0 +1,178 @@
+// mcx_ctx.h -- the handle behind the C ABI (one simulated system on one
+// GPU) and the host helpers shared by the single-system and batched entry
+// points.
+#pragma once
+
+#include "../../include/mcx.h"
+#include "mcx_internal.h"
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count)
+    {
+        if (count <= n && p) return;
+        if (p) HIPCHK(hipFree(p));
+        p = nullptr;
+        n = 0;
+        if (count == 0) count = 1;
+        HIPCHK(hipMalloc((void **)&p, count * sizeof(T)));
+        n = count;
+    }
+    void upload(const T *h, size_t count, hipStream_t st)
+    {
+        alloc(count);
+        if (count) HIPCHK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, st));
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+// Device buffers of a move set (instance table and term records).
+struct MoveBufs {
+    DBuf<int4> rid;
+    DBuf<int> rkk, rty;
+    DBuf<double4> rpar;
+    DBuf<int2> inst_adj, inst_pairs, pair_ent;
+    DBuf<int4> inst;
+    void release()
+    {
+        rid.release(); rkk.release(); rty.release(); rpar.release(); inst_adj.release(); inst.release();
+        inst_pairs.release(); pair_ent.release();
+    }
+};
+
+struct mcx_ctx {
+    int n = 0, device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // parameters
+    int pbtype = MCX_PBC_CUBIC;
+    double box[3] = {0, 0, 0};
+    int cutmode = MCX_CUT_SWITCH;
+    double ctoff = 12.0, cton = 10.0, dcut = 2.0, temp = 300.0;
+    bool have_atoms = false, have_coords = false, have_excl = false, list_valid = false;
+    std::vector<double> q, eps, sig, eps14, sig14, mass;
+    std::vector<int> frozen, resid;
+    HostTerms terms;    // kinds 0..4 from the caller, 5/6 from the exclusion builder
+    std::vector<int> excl_pair, excl_off, excl_idx;   // visible rows
+    std::vector<int> pairs14;
+    // device buffers
+    DBuf<double> x, y, z, q64, eps64, sig64, eps1464, sig1464, mass64;
+    DBuf<int4> ipos;
+    DBuf<float2> es, esk, es14;
+    DBuf<double4> qes;
+    std::vector<double> hx, hy, hz;   // host copy of the last uploaded coordinates
+    DBuf<int> frozen_d, excl_off_d, excl_idx_d, xfull_off_d, xfull_idx_d;
+    DBuf<double4> cpos;
+    DBuf<int> nbr, cnt, half, overflow;
+    int cap = 0;
+    DBuf<int> cell_of, cell_start, cell_count, cell_atoms, cursor;
+    CellGrid grid{};
+    DBuf<int2> tiles;
+    int ntiles = 0;
+    DBuf<double> partial, result;
+    DBuf<int4> t_idx[kNumKinds];
+    DBuf<int> t_type[kNumKinds];
+    DBuf<double4> t_par[kNumKinds];
+    DBuf<int> adj_off, adj_kk, adj_type;
+    DBuf<int4> adj_ids;
+    DBuf<double4> adj_par;
+    std::vector<int> adj_off_h, adj_kk_h;
+    std::vector<int> xfull_off_h, xfull_idx_h;   // excluded partners, both directions (CSR)
+    MoveBufs mvb;
+    uint32_t rng_stream = 0;   // Philox stream of mcx_mc_run
+    DBuf<double> newxyz, scalar;
+    long long last_npairs = 0;
+
+    DevSys sys() const
+    {
+        DevSys s{};
+        s.n = n;
+        s.pbtype = pbtype;
+        for (int k = 0; k < 3; k++) {
+            s.L[k] = box[k];
+            s.rL[k] = pbtype == MCX_PBC_NONE ? 0.0 : 1.0 / box[k];
+            s.Lf[k] = (float)s.L[k];
+            s.rLf[k] = (float)s.rL[k];
+        }
+        s.ctoff2 = ctoff * ctoff;
+        s.cton2 = cton * cton;
+        const double dd = s.ctoff2 - s.cton2;
+        s.switch2 = 1. / (dd * dd * dd);
+        double cn = ctoff + dcut;
+        s.cutnb2 = cn * cn;
+        s.cutmode = cutmode;
+        s.x = x.p; s.y = y.p; s.z = z.p;
+        s.ipos = ipos.p;
+        s.es = es.p; s.esk = esk.p; s.es14 = es14.p;
+        for (int k = 0; k < 3; k++) s.unit[k] = pbtype == MCX_PBC_NONE ? 1.0 : box[k] / 4294967296.0;
+        s.iunit = pbtype == MCX_PBC_CUBIC ? 4294967296.0 / box[0] : 1.0;
+        s.qes = qes.p;
+        s.q64 = q64.p; s.eps64 = eps64.p; s.sig64 = sig64.p; s.eps1464 = eps1464.p; s.sig1464 = sig1464.p;
+        s.frozen = frozen_d.p;
+        s.excl_off = excl_off_d.p; s.excl_idx = excl_idx_d.p;
+        s.xfull_off = xfull_off_d.p; s.xfull_idx = xfull_idx_d.p;
+        s.cap = cap;
+        s.nbr = nbr.p; s.cnt = cnt.p; s.half = half.p; s.overflow = overflow.p;
+        return s;
+    }
+
+    DevTerms dterms() const
+    {
+        DevTerms t{};
+        t.start[0] = 0;
+        for (int k = 0; k < kNumKinds; k++) {
+            t.nterm[k] = (int)terms.idx[k].size();
+            t.start[k + 1] = t.start[k] + t.nterm[k];
+            t.idx[k] = t_idx[k].p;
+            t.type[k] = t_type[k].p;
+            t.par[k] = t_par[k].p;
+        }
+        // the excluded-pair correction applies to FULL mode only
+        if (cutmode != MCX_CUT_FULL) {
+            t.nterm[kExclPair] = 0;
+            t.start[kExclPair + 1] = t.start[kExclPair];
+        }
+        return t;
+    }
+
+    ~mcx_ctx()
+    {
+        DBuf<double> *d[] = {&x, &y, &z, &q64, &eps64, &sig64, &eps1464, &sig1464, &mass64, &partial, &result,
+                             &newxyz, &scalar};
+        for (auto *b : d) b->release();
+        ipos.release(); es.release(); esk.release(); es14.release(); qes.release();
+        cpos.release();
+        adj_ids.release();
+        adj_par.release();
+        adj_type.release();
+        mvb.release();
+        DBuf<int> *ib[] = {&frozen_d, &excl_off_d, &excl_idx_d, &xfull_off_d, &xfull_idx_d, &nbr, &cnt, &half, &overflow, &cell_of,
+                           &cell_start, &cell_count, &cell_atoms, &cursor, &adj_off, &adj_kk};
+        for (auto *b : ib) b->release();
+        tiles.release();
+        for (int k = 0; k < kNumKinds; k++) { t_idx[k].release(); t_type[k].release(); t_par[k].release(); }
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+
+// host helpers (mcx_api.cpp)
+namespace mcxi {
+long long build_list(mcx_ctx *h);
+void energy_device(mcx_ctx *h, double out9[9]);
+void assemble(const double r[9], double ener[10]);
+void sync_host_coords(mcx_ctx *h);
+DevMoves build_moves(mcx_ctx *h, const mcx_move_type *types, int ntypes, MoveBufs &buf, int max_atoms);
+void mt_seed(uint32_t *st, uint64_t seed);
+}  // namespace mcxi

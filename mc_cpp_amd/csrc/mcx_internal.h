@@ -29,6 +29,10 @@
 
 constexpr int kMaxMoveAtoms = 64;   // atoms per move instance on the device path
 constexpr int kMaxMoveTypes = 8;
+// Saved draw state of a chain (uint32 words): [0,624) MT19937 state, 624 MT
+// index, 625/626 Philox block counter (hi, lo), 627 cursor into the draw
+// batch, [628,692) the batch (32 doubles).
+constexpr int kRngWords = 692;
 
 // Everything a kernel needs about one system, passed by value.
 struct DevSys {
@@ -50,6 +54,7 @@ struct DevSys {
     float2 *esk;         // (eps, sig) FP32, sig in kernel length units
     float2 *es14;        // (eps14, sig14)
     const double *q64, *eps64, *sig64, *eps1464, *sig1464;
+    const double4 *qes;  // (charge, eps, sig in A, 0) FP64 per atom: the per-move dE path
     const int *frozen;
     // exclusion rows (visible entries, CSR)
     const int *excl_off, *excl_idx;
@@ -87,6 +92,11 @@ struct DevMoves {
     int each[kMaxMoveTypes];
     const int4 *inst;                  // flat instance table: (first, last, pivot, 0)
     const int2 *inst_adj;              // range of the instance's term records
+    const int2 *inst_pairs;            // range of the instance's pair corrections in pair_ent
+    // pair corrections of a list-free dE: (a, j) = moved atom a (offset in
+    // the range) and an excluded partner j outside it, to subtract; or
+    // (a | 1 << 16, b) = a non-excluded pair of moved atoms a < b, to add
+    const int2 *pair_ent;
     // term records: every bonded term / 1-4 pair touching an instance, once
     const int4 *rec_ids;
     const int *rec_kk;                 // kind << 28 | index within kind
@@ -148,6 +158,7 @@ void launch_delta_energy(const DevSys &s, const DevTerms &t, const DevMoves &mv,
 struct MCRunArgs {
     int nsteps, step0, rng_mode;
     uint64_t seed;
+    uint32_t stream;       // Philox stream id
     double beta;
     double *etot;          // device scalar
     uint32_t *mt_state;    // [692] device: MT19937 state, index, Philox counter, draw batch
@@ -160,3 +171,50 @@ void launch_mc_chunk(const DevSys &s, const DevTerms &t, const DevMoves &mv, con
                      hipStream_t st);
 // adjoint.hip
 void launch_nonbonded_grad(const DevSys &s, double *g6, hipStream_t st);
+// batch.hip: one Metropolis chain per replica, one workgroup each
+constexpr int kBatchTypes = 64;     // distinct (q, eps, sig) atom types of a replica
+constexpr int kBatchAtoms = 32;     // atoms per move instance
+struct BatchRep {
+    DevSys s;
+    DevMoves mv;
+    const int *atype;        // [n] type index | frozen << 16
+    const double4 *ptab;     // [ntab] (charge, eps, sig in A, 0)
+    int ntab;
+    int sweep_len;           // moves per sweep
+    int frozen_any;
+    int rng_mode;            // 1: std::mt19937 replay, 0: Philox
+    uint64_t key;            // Philox key
+    uint32_t stream;         // Philox stream
+    uint32_t *rng;           // [692] draw state
+    double *etot;            // running total energy (iu)
+    double *dmax;            // [kMaxMoveTypes]
+    long long *trials, *accepted;   // [kMaxMoveTypes]
+    int *acc_tmp;            // [kMaxMoveTypes]
+    long long *step;         // moves done since the chain was (re)started
+};
+struct BatchArgs {
+    const BatchRep *reps;
+    const int *order;            // workgroup -> replica
+    const double *temps;         // [nrep] temperature of each ladder slot
+    const int *slot_of_rep;      // [nrep]
+    int img_cap;                 // atoms held by the LDS image
+    int nmoves;                  // moves per replica in this launch (<= 0: the replica's sweep_len)
+    int rec_len;                 // moves recorded per replica (the launch's first rec_len)
+    unsigned char *acc_out;      // [nrep * rec_len] or null
+    double *de_out;              // [nrep * rec_len] or null
+};
+struct ExchArgs {
+    int nrep;
+    uint64_t key;
+    const double *temps;         // [nrep] slot temperatures
+    const double *etot;          // [nrep] replica energies
+    int *slot_of_rep, *rep_of_slot;
+    unsigned long long *counter; // exchange rounds so far
+    long long *att, *acc;        // [nrep - 1] per neighbouring slot pair
+};
+bool batch_variant_ok(int threads, int unroll, bool img);
+size_t batch_static_lds(int threads, int unroll, bool img);
+size_t batch_dynamic_lds(int threads, bool img, int img_cap);
+void batch_prepare(int threads, int unroll, bool img, int img_cap);   // sets the kernel's LDS limit
+void launch_batch_sweep(const BatchArgs &a, int nrep, int threads, int unroll, bool img, hipStream_t st);
+void launch_exchange(const ExchArgs &x, hipStream_t st);

@@ -236,20 +236,22 @@ __device__ inline void mcx_term_energy_rec(const DevSys &s, int kind, const int4
             evdw = lj == 0.f || ej.x == 0.f ? 0.0 : -((double)ei.x * ((double)ej.x * (double)lj));
             return;
         }
-        const float2 ei = s.es14[id.x];
-        const float2 ej = s.es14[id.y];
+        // FP64 throughout: the per-move dE differences this value
         double d[3] = {dj[0] - di[0], dj[1] - di[1], dj[2] - di[2]};
         mcx_mimg64(s, d);
-        double r2, c;
-        float v;
-        mcx_pair(d[0], d[1], d[2], (float)s.q64[id.y], ej.x, ei.y + ej.y, r2, c, v);
+        const double r2 = fma(d[2], d[2], fma(d[1], d[1], d[0] * d[0]));
         double sw = 1.0;
         if (s.cutmode == 1) {
             if (!(r2 <= s.ctoff2)) return;
             sw = mcx_switch(s, r2);
         }
-        eel = sw * ((double)(float)s.q64[id.x] * c);
-        evdw = sw * ((double)ei.x * (double)v);
+        float ri32;
+        const double ri = mcx_rsqrt(r2, ri32);
+        const double sij = s.sig1464[id.x] + s.sig1464[id.y];
+        const double t = sij * sij * (ri * ri);
+        const double t3 = t * t * t;
+        eel = sw * (s.q64[id.x] * (s.q64[id.y] * ri));
+        evdw = sw * (s.eps1464[id.x] * (s.eps1464[id.y] * fma(t3, t3, -t3)));
     }
 }
 

@@ -27,6 +27,7 @@ MCX_OK = 0
 ERRORS = {1: "MCX_ERR_ARG", 2: "MCX_ERR_HIP", 3: "MCX_ERR_LIST_OVERFLOW", 4: "MCX_ERR_BOX_TOO_SMALL",
           5: "MCX_ERR_STATE", 6: "MCX_ERR_UNSUPPORTED"}
 MC_REPLAY, MC_PHILOX = 1, 0
+MAX_MOVE_TYPES = 8
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -47,6 +48,13 @@ class MCStatsC(C.Structure):
     _fields_ = [("etot", C.c_double), ("trials", C.c_longlong * 8), ("accepted", C.c_longlong * 8),
                 ("dmax", C.c_double * 8), ("list_updates", C.c_longlong), ("seconds", C.c_double)]
 
+
+class BatchStatsC(C.Structure):
+    _fields_ = [("moves", C.c_longlong), ("accepted", C.c_longlong), ("exchange_attempts", C.c_longlong),
+                ("exchange_accepted", C.c_longlong), ("sweeps", C.c_longlong), ("seconds", C.c_double)]
+
+
+_llp = C.POINTER(C.c_longlong)
 
 # every symbol include/mcx.h declares, with its argument types
 SIGNATURES = {
@@ -75,6 +83,20 @@ SIGNATURES = {
     "mcx_nonbonded_grad": ([C.c_void_p] + [_dp] * 6, C.c_int),
     "mcx_device_coords": ([C.c_void_p, C.c_int, C.POINTER(C.c_void_p)], C.c_int),
     "mcx_time_kernel": ([C.c_void_p, C.c_int, C.c_int, _dp], C.c_int),
+    "mcx_set_rng_stream": ([C.c_void_p, C.c_uint32], C.c_int),
+    "mcx_batch_create": ([C.POINTER(C.c_void_p), C.c_int, C.POINTER(MoveTypeC), _ip, C.POINTER(C.c_void_p)],
+                         C.c_int),
+    "mcx_batch_destroy": ([C.c_void_p], None),
+    "mcx_batch_last_error": ([C.c_void_p], C.c_char_p),
+    "mcx_batch_reset": ([C.c_void_p, C.c_uint64, C.c_int, C.POINTER(C.c_uint64)], C.c_int),
+    "mcx_batch_set_ladder": ([C.c_void_p, _dp], C.c_int),
+    "mcx_batch_set_sweep": ([C.c_void_p, _ip], C.c_int),
+    "mcx_batch_run": ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(BatchStatsC)], C.c_int),
+    "mcx_batch_moves": ([C.c_void_p, C.c_int, C.POINTER(C.c_ubyte), _dp], C.c_int),
+    "mcx_batch_state": ([C.c_void_p, _dp, _ip, _llp, _llp, _dp], C.c_int),
+    "mcx_batch_exchange_stats": ([C.c_void_p, _llp, _llp], C.c_int),
+    "mcx_batch_set_config": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
+    "mcx_batch_tune": ([C.c_void_p, _ip, _ip, _dp], C.c_int),
 }
 
 _lib = None
@@ -229,14 +251,12 @@ class Handle:
         self._chk(self.L.mcx_time_kernel(self.h, int(which), int(reps), C.byref(ms)))
         return ms.value
 
+    def set_rng_stream(self, stream: int):
+        self._chk(self.L.mcx_set_rng_stream(self.h, C.c_uint32(stream)))
+
     def mc_run(self, moves: Sequence[S.MoveType], nsteps, seed, rng_mode=MC_REPLAY, list_every=50, record=True):
         nt = len(moves)
-        arr = (MoveTypeC * nt)()
-        keep = []
-        for k, m in enumerate(moves):
-            arr[k].type, arr[k].mode, arr[k].dmax, arr[k].target, arr[k].each = m.type, m.mode, m.dmax, m.target, m.each
-            if m.sele is not None:
-                sa = _i(m.sele); keep.append(sa); arr[k].sele = _p(sa, C.c_int)
+        arr, keep = _move_array(moves)
         st = MCStatsC()
         acc = np.zeros(max(nsteps, 1), np.uint8) if record else None
         de = np.zeros(max(nsteps, 1)) if record else None
@@ -250,6 +270,16 @@ class Handle:
         if record:
             out.update(acc=acc[:nsteps], de=de[:nsteps], mvtyp=ty[:nsteps], mvinst=ins[:nsteps])
         return out
+
+
+def _move_array(moves: Sequence[S.MoveType]):
+    arr = (MoveTypeC * len(moves))()
+    keep = []
+    for k, m in enumerate(moves):
+        arr[k].type, arr[k].mode, arr[k].dmax, arr[k].target, arr[k].each = m.type, m.mode, m.dmax, m.target, m.each
+        if m.sele is not None:
+            sa = _i(m.sele); keep.append(sa); arr[k].sele = _p(sa, C.c_int)
+    return arr, keep
 
 
 def from_system(s: S.System, device: int = 0) -> Handle:
@@ -335,3 +365,100 @@ class MC_metropolis:
         if self.nsteps == 0:
             return None       # "Dummy MC simulation because nsteps = 0" (src/MC.cpp:177-181)
         return self.ff.h.mc_run(self.moves, self.nsteps, self.seed, self.rng_mode, self.list_every, record)
+
+
+class Batch:
+    """Replicas run together: one Metropolis chain per member handle, one
+    persistent kernel launch per sweep, replica exchange on the device
+    (mcx_batch_* of include/mcx.h).  The handles must stay alive as long as
+    the batch; the batch keeps references to them."""
+
+    def __init__(self, handles: Sequence[Handle], moves):
+        """moves: one list of MoveType shared by every replica (selections
+        None), or one list per replica."""
+        self.L = load_library()
+        self.handles = list(handles)
+        self.nrep = len(self.handles)
+        per = [list(m) for m in moves] if len(moves) and isinstance(moves[0], (list, tuple)) \
+            else [list(moves)] * self.nrep
+        assert len(per) == self.nrep
+        self.ntypes = np.array([len(m) for m in per], dtype=np.int32)
+        hs = (C.c_void_p * self.nrep)(*[h.h for h in self.handles])
+        arr, keep = _move_array([m for ms in per for m in ms])
+        b = C.c_void_p()
+        rc = self.L.mcx_batch_create(hs, self.nrep, arr, _p(self.ntypes, C.c_int), C.byref(b))
+        if rc != MCX_OK:
+            raise MCXError(rc, self.L.mcx_last_error(self.handles[0].h).decode())
+        self.b = b
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.L.mcx_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != MCX_OK:
+            raise MCXError(rc, self.L.mcx_batch_last_error(self.b).decode())
+
+    def reset(self, seed=0, rng_mode=MC_PHILOX, seeds=None):
+        sp = None
+        if seeds is not None:
+            sa = np.ascontiguousarray(seeds, dtype=np.uint64)
+            sp = sa.ctypes.data_as(C.POINTER(C.c_uint64))
+        self._chk(self.L.mcx_batch_reset(self.b, C.c_uint64(seed), int(rng_mode), sp))
+
+    def set_ladder(self, temps):
+        t = _d(temps)
+        assert t.size == self.nrep
+        self._chk(self.L.mcx_batch_set_ladder(self.b, _p(t, C.c_double)))
+
+    def set_sweep(self, moves_per_sweep):
+        m = _i(moves_per_sweep)
+        assert m.size == self.nrep
+        self._chk(self.L.mcx_batch_set_sweep(self.b, _p(m, C.c_int)))
+
+    def run(self, nsweeps, exchange_every=0, use_graph=True):
+        st = BatchStatsC()
+        self._chk(self.L.mcx_batch_run(self.b, int(nsweeps), int(exchange_every), int(bool(use_graph)), C.byref(st)))
+        return dict(moves=st.moves, accepted=st.accepted, exchange_attempts=st.exchange_attempts,
+                    exchange_accepted=st.exchange_accepted, sweeps=st.sweeps, seconds=st.seconds)
+
+    def moves(self, nmoves, record=True):
+        acc = np.zeros((self.nrep, nmoves), np.uint8) if record else None
+        de = np.zeros((self.nrep, nmoves)) if record else None
+        self._chk(self.L.mcx_batch_moves(self.b, int(nmoves), _p(acc, C.c_ubyte), _p(de, C.c_double)))
+        return acc, de
+
+    def state(self):
+        """trials / accepted / dmax: (nrep, MAX_MOVE_TYPES), zero past each
+        replica's own move types."""
+        R, T = self.nrep, MAX_MOVE_TYPES
+        etot = np.zeros(R)
+        slot = np.zeros(R, np.int32)
+        tr = np.zeros((R, T), np.int64)
+        ac = np.zeros((R, T), np.int64)
+        dm = np.zeros((R, T))
+        self._chk(self.L.mcx_batch_state(self.b, _p(etot, C.c_double), _p(slot, C.c_int), _p(tr, C.c_longlong),
+                                         _p(ac, C.c_longlong), _p(dm, C.c_double)))
+        return dict(etot=etot, slot=slot, trials=tr, accepted=ac, dmax=dm)
+
+    def exchange_stats(self):
+        n = max(self.nrep - 1, 1)
+        a = np.zeros(n, np.int64)
+        c = np.zeros(n, np.int64)
+        self._chk(self.L.mcx_batch_exchange_stats(self.b, _p(a, C.c_longlong), _p(c, C.c_longlong)))
+        return a[: self.nrep - 1], c[: self.nrep - 1]
+
+    def set_config(self, threads, unroll):
+        self._chk(self.L.mcx_batch_set_config(self.b, int(threads), int(unroll)))
+
+    def tune(self):
+        t, u, ms = C.c_int(0), C.c_int(0), C.c_double(0)
+        self._chk(self.L.mcx_batch_tune(self.b, C.byref(t), C.byref(u), C.byref(ms)))
+        return t.value, u.value, ms.value

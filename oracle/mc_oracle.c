@@ -557,6 +557,56 @@ static int lower_bound_row(const int *row, int n, int v)
     return lo;
 }
 
+/* 1-4 pairs and bonded terms touching [first,last] (used by both forms of
+ * the moving-set energy below); returns elec_raw/vdw_raw of the 1-4 pairs
+ * through lelec and levdw and the bonded sum as the result. */
+static double moving_set_rest(orc_system *s, int first, int last, double *lelec, double *levdw)
+{
+#define INSET(a) ((a) >= first && (a) <= last)
+    if (s->cutmode == ORC_CUT_SWITCH) {
+        const double ctoff2 = s->ctoff * s->ctoff, cton2 = s->cton * s->cton;
+        const double switch2 = 1. / (X3(ctoff2 - cton2));
+        for (int k = 0; k < s->nPair14; k++) {
+            int i = s->neighList14[2 * k], j = s->neighList14[2 * k + 1];
+            if (!INSET(i) && !INSET(j)) continue;
+            double di[3], dj[3];
+            getc3(s, i, di); getc3(s, j, dj);
+            double r2 = distance2(s, di, dj);
+            if (r2 <= ctoff2) {
+                double pe, pv;
+                switched_pair(s, r2, s->charge[i], s->charge[j], s->eps14[i], s->eps14[j],
+                              s->sig14[i], s->sig14[j], ctoff2, cton2, switch2, &pe, &pv);
+                *lelec += pe; *levdw += pv;
+            }
+        }
+    } else {
+        for (int k = 0; k < s->nPair14; k++) {
+            int i = s->neighList14[2 * k], j = s->neighList14[2 * k + 1];
+            if (!INSET(i) && !INSET(j)) continue;
+            double di[3], dj[3];
+            getc3(s, i, di); getc3(s, j, dj);
+            double rt = 1. / sqrt(distance2(s, di, dj));
+            *lelec += e_elec(s->charge[i], s->charge[j], rt);
+            *levdw += e_vdw(s->eps14[i], s->eps14[j], s->sig14[i], s->sig14[j], rt);
+        }
+    }
+    double e = 0.;
+    for (int l = 0; l < s->nbond; l++) if (INSET(s->bond_a[l]) || INSET(s->bond_b[l])) e += bond_term(s, l);
+    for (int l = 0; l < s->nangle; l++)
+        if (INSET(s->ang_a[l]) || INSET(s->ang_b[l]) || INSET(s->ang_c[l])) e += angle_term(s, l);
+    for (int l = 0; l < s->nub; l++) if (INSET(s->ub_a[l]) || INSET(s->ub_b[l])) e += ub_term(s, l);
+    for (int l = 0; l < s->ndihe; l++)
+        if (INSET(s->dih_a[l]) || INSET(s->dih_b[l]) || INSET(s->dih_c[l]) || INSET(s->dih_d[l]))
+            e += torsion_term(s, s->dih_a[l], s->dih_b[l], s->dih_c[l], s->dih_d[l], s->dih_type[l],
+                              s->dih_k[l], s->dih_phi0[l], s->dih_mult[l]);
+    for (int l = 0; l < s->nimpr; l++)
+        if (INSET(s->imp_a[l]) || INSET(s->imp_b[l]) || INSET(s->imp_c[l]) || INSET(s->imp_d[l]))
+            e += torsion_term(s, s->imp_a[l], s->imp_b[l], s->imp_c[l], s->imp_d[l], s->imp_type[l],
+                              s->imp_k[l], s->imp_phi0[l], s->imp_mult[l]);
+#undef INSET
+    return e;
+}
+
 double orc_moving_set_energy(orc_system *s, int first, int last)
 {
     const int N = s->natom;
@@ -588,19 +638,6 @@ double orc_moving_set_energy(orc_system *s, int first, int last)
                 }
             }
         }
-        for (int k = 0; k < s->nPair14; k++) {
-            int i = s->neighList14[2 * k], j = s->neighList14[2 * k + 1];
-            if (!INSET(i) && !INSET(j)) continue;
-            double di[3], dj[3];
-            getc3(s, i, di); getc3(s, j, dj);
-            double r2 = distance2(s, di, dj);
-            if (r2 <= ctoff2) {
-                double pe, pv;
-                switched_pair(s, r2, s->charge[i], s->charge[j], s->eps14[i], s->eps14[j],
-                              s->sig14[i], s->sig14[j], ctoff2, cton2, switch2, &pe, &pv);
-                lelec += pe; levdw += pv;
-            }
-        }
     } else {
         for (int i = 0; i < N; i++) {
             for (int j = i + 1; j < N; j++) {
@@ -613,31 +650,45 @@ double orc_moving_set_energy(orc_system *s, int first, int last)
                 levdw += e_vdw(s->eps[i], s->eps[j], s->sig[i], s->sig[j], rt);
             }
         }
-        for (int k = 0; k < s->nPair14; k++) {
-            int i = s->neighList14[2 * k], j = s->neighList14[2 * k + 1];
-            if (!INSET(i) && !INSET(j)) continue;
-            double di[3], dj[3];
-            getc3(s, i, di); getc3(s, j, dj);
-            double rt = 1. / sqrt(distance2(s, di, dj));
-            lelec += e_elec(s->charge[i], s->charge[j], rt);
-            levdw += e_vdw(s->eps14[i], s->eps14[j], s->sig14[i], s->sig14[j], rt);
+    }
+#undef INSET
+    const double eb = moving_set_rest(s, first, last, &lelec, &levdw);
+    return CHGCHARMM * KCALTOIU * lelec + 4.0 * levdw + eb;
+}
+
+/* The same energy with the pair list replaced by its definition at the
+ * current coordinates: every pair (i in [first,last], any j) that
+ * update_verlet_list (src/List_nonBonded.cpp:836-915) would list -- not
+ * excluded, not both frozen -- and whose distance is inside the cutoff.
+ * Equal to orc_moving_set_energy right after a list update; it defines the
+ * dE of the batched replica path, which keeps no list.  Not in the
+ * reference.  FULL mode has no list: same as orc_moving_set_energy. */
+double orc_moving_set_energy_exact(orc_system *s, int first, int last)
+{
+    if (s->cutmode != ORC_CUT_SWITCH) return orc_moving_set_energy(s, first, last);
+    const int N = s->natom;
+    const double ctoff2 = s->ctoff * s->ctoff, cton2 = s->cton * s->cton;
+    const double switch2 = 1. / (X3(ctoff2 - cton2));
+    double lelec = 0., levdw = 0.;
+    for (int i = first; i <= last; i++) {
+        double di[3], dj[3];
+        getc3(s, i, di);
+        for (int j = 0; j < N; j++) {
+            if (j == i || (j >= first && j <= last && j < i)) continue;
+            const int lo = i < j ? i : j, hi = i < j ? j : i;
+            if ((s->frozen[lo] && s->frozen[hi]) || excl_member(s, lo, hi)) continue;
+            getc3(s, j, dj);
+            double r2 = distance2(s, di, dj);
+            if (r2 <= ctoff2) {
+                double pe, pv;
+                switched_pair(s, r2, s->charge[i], s->charge[j], s->eps[i], s->eps[j], s->sig[i], s->sig[j], ctoff2,
+                              cton2, switch2, &pe, &pv);
+                lelec += pe; levdw += pv;
+            }
         }
     }
-    double e = CHGCHARMM * KCALTOIU * lelec + 4.0 * levdw;
-    for (int l = 0; l < s->nbond; l++) if (INSET(s->bond_a[l]) || INSET(s->bond_b[l])) e += bond_term(s, l);
-    for (int l = 0; l < s->nangle; l++)
-        if (INSET(s->ang_a[l]) || INSET(s->ang_b[l]) || INSET(s->ang_c[l])) e += angle_term(s, l);
-    for (int l = 0; l < s->nub; l++) if (INSET(s->ub_a[l]) || INSET(s->ub_b[l])) e += ub_term(s, l);
-    for (int l = 0; l < s->ndihe; l++)
-        if (INSET(s->dih_a[l]) || INSET(s->dih_b[l]) || INSET(s->dih_c[l]) || INSET(s->dih_d[l]))
-            e += torsion_term(s, s->dih_a[l], s->dih_b[l], s->dih_c[l], s->dih_d[l], s->dih_type[l],
-                              s->dih_k[l], s->dih_phi0[l], s->dih_mult[l]);
-    for (int l = 0; l < s->nimpr; l++)
-        if (INSET(s->imp_a[l]) || INSET(s->imp_b[l]) || INSET(s->imp_c[l]) || INSET(s->imp_d[l]))
-            e += torsion_term(s, s->imp_a[l], s->imp_b[l], s->imp_c[l], s->imp_d[l], s->imp_type[l],
-                              s->imp_k[l], s->imp_phi0[l], s->imp_mult[l]);
-#undef INSET
-    return e;
+    const double eb = moving_set_rest(s, first, last, &lelec, &levdw);
+    return CHGCHARMM * KCALTOIU * lelec + 4.0 * levdw + eb;
 }
 
 /* ------------------------------------------------------------------ */
@@ -845,7 +896,9 @@ double orc_mc_run(orc_system *s, const orc_movetype *types, int ntypes,
         int imvtyp = (int)((double)ntypes * u_alpha(&g));
         int imvatm = (int)((double)ninst[imvtyp] * u_alpha(&g));
         const orc_inst in = inst[imvtyp][imvatm];
-        double eold = (mode == 0) ? orc_getE(s, NULL) : orc_moving_set_energy(s, in.first, in.last);
+        double eold = mode == 0 ? orc_getE(s, NULL)
+                      : mode == 2 ? orc_moving_set_energy_exact(s, in.first, in.last)
+                                  : orc_moving_set_energy(s, in.first, in.last);
         for (int i = in.first; i <= in.last; i++) { bx[i] = s->x[i]; by[i] = s->y[i]; bz[i] = s->z[i]; }
         double r[3];
         do { r[0] = u_move(&g); r[1] = u_move(&g); r[2] = u_move(&g); }   /* rndSphere :142-156 */
@@ -862,7 +915,9 @@ double orc_mc_run(orc_system *s, const orc_movetype *types, int ntypes,
             else centre_of_mass(s, in.first, in.last, piv);
             rotate_range(s, piv, R, in.first, in.last);
         }
-        double enew = (mode == 0) ? orc_getE(s, NULL) : orc_moving_set_energy(s, in.first, in.last);
+        double enew = mode == 0 ? orc_getE(s, NULL)
+                      : mode == 2 ? orc_moving_set_energy_exact(s, in.first, in.last)
+                                  : orc_moving_set_energy(s, in.first, in.last);
         double de = enew - eold;
         int acc;
         if (de <= DBL_EPSILON) acc = 1;

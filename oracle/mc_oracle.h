@@ -91,6 +91,8 @@ void orc_eimpr(orc_system *s);
 /* energy of every nonbonded (list or all-pair), 1-4 and bonded term that
  * touches at least one atom of [first,last]; used for dE = E_set(new)-E_set(old) */
 double orc_moving_set_energy(orc_system *s, int first, int last);
+/* the same with every pair the list would hold at the current coordinates */
+double orc_moving_set_energy_exact(orc_system *s, int first, int last);
 
 /* MT19937 (C++ [rand.eng.mers]) + libstdc++ generate_canonical<double,53> */
 typedef struct orc_mt { uint32_t mt[624]; int idx; } orc_mt;
@@ -114,8 +116,10 @@ typedef struct orc_mc_record {
 } orc_mc_record;
 
 /* Runs the Metropolis loop of MC_metropolis::run.  mode 0 = reference
- * compat (two full getE per move), 1 = moving-set dE.  Returns final etot;
- * final dmax per type written to dmax_out. */
+ * compat (two full getE per move), 1 = moving-set dE, 2 = moving-set dE
+ * over every pair the list would hold at the current coordinates (the
+ * batched path's definition; equals mode 1 until an atom outruns the list
+ * skin).  Returns final etot; final dmax per type written to dmax_out. */
 double orc_mc_run(orc_system *s, const orc_movetype *types, int ntypes,
                   int nsteps, uint64_t seed, int mode, int list_every,
                   orc_mc_record *rec, double *dmax_out, int nthreads);

@@ -82,6 +82,8 @@ def lib():
         L.orc_getE.restype = C.c_double
         L.orc_moving_set_energy.argtypes = [P, C.c_int, C.c_int]
         L.orc_moving_set_energy.restype = C.c_double
+        L.orc_moving_set_energy_exact.argtypes = [P, C.c_int, C.c_int]
+        L.orc_moving_set_energy_exact.restype = C.c_double
         L.orc_free_lists.argtypes = [P]
         L.orc_set_box.argtypes = [P, C.c_int, C.c_double, C.c_double, C.c_double]
         L.orc_mc_run.argtypes = [P, C.POINTER(OrcMoveType), C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int,
@@ -201,6 +203,10 @@ class Oracle:
 
     def moving_set_energy(self, first, last):
         return self.L.orc_moving_set_energy(C.byref(self.s), int(first), int(last))
+
+    def moving_set_energy_exact(self, first, last):
+        """Every pair the list would hold at the current coordinates."""
+        return self.L.orc_moving_set_energy_exact(C.byref(self.s), int(first), int(last))
 
     def grad_switch(self):
         n = self.sys.natom

@@ -76,7 +76,10 @@ def main():
     # synthetic systems
     synth = {"argon_fcc_seed1": systems.argon_fcc(seed=1),
              "water512_seed3": systems.water_box(seed=3, nside=8, L=24.85),
-             "water4096_seed2": systems.water_box(seed=2)}
+             "water4096_seed2": systems.water_box(seed=2),
+             # a box smaller than twice the cutoff (replica-ladder sizes)
+             "water90_small_box": systems.water_box(seed=21, nside=5, L=(90 / 0.0334) ** (1 / 3), nmol=90,
+                                                    temp=290.0, tag=300)}
     for name, (s, fp) in synth.items():
         cor, ff = systems.save_mdbas(os.path.join(tmp, name), s, fp)
         r = oracle.Reference(cor, ff, s)
@@ -86,7 +89,7 @@ def main():
         ent["energy_full"] = oracle.Reference(cor2, ff2, s2).getE().tolist()
         if name != "water4096_seed2":
             nsteps = 1000 if name.startswith("argon") else 300
-            seed = 12345 if name.startswith("argon") else 777
+            seed = 12345 if name.startswith("argon") else 11 if name == "water90_small_box" else 777
             mc = r.mc_run(s.moves, nsteps, seed)
             x, y, z = r.coords()
             ent["mc"] = {"nsteps": nsteps, "seed": seed, "trials": mc["trials"].tolist(),

@@ -1,0 +1,242 @@
+"""Batched replicas (one persistent kernel launch per sweep, replica exchange
+on the device) against the FP64 oracle and against the single-chain device
+loop, through the C ABI.
+
+The batched chains count every pair inside the cutoff; the reference (and
+the single-chain loop) count the pairs of a Verlet list built up to 50
+moves earlier.  The two agree until an atom outruns the 2 A list skin,
+which residue rotations can do: the reference rotates a residue about the
+move's atom (include/Move.hpp:120-143) and, by its residue lookup quirk
+(src/List_Moves.cpp:659-685), that atom may belong to the neighbouring
+residue, several A away, so one accepted rotation can carry a molecule by
+more than the skin.  The oracle's list-free mode (oracle mode 2, pinned
+against brute force and against the list mode in tests/test_oracle.py) is
+therefore the reference for the batched chains.  Against the list-based
+single-chain loop, dE is compared on moves with |dE| < 5e4 iu (a jump that
+tears a molecule across the box edge stretches an O-H bond past 1 A, over
+9e4 iu) and accept flags on every move.
+
+Tolerances (stated per quantity):
+  * per-move dE: |dE_batch - dE_ref| <= 1e-5 * max(|dE|, kT), the bar of the
+    single-chain tests; against the single-chain device loop, which shares
+    the per-pair arithmetic, 1e-9 * max(|dE|, kT);
+  * accept sequences: identical, except a move whose two dE straddle the
+    acceptance threshold within that band (the chains then diverge);
+  * running total after a run vs a fresh energy of the final coordinates:
+    relative 1e-6 (the bar for total energies);
+  * replica-exchange decisions: exact, recomputed on the host from the
+    device energies and the same Philox4x32-10 draws;
+  * hipGraph replay vs plain launches, and every launch shape: identical
+    accept sequences and energies.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from mc_cpp_amd import mcx, philox, systems
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+KB_REF = 0.83144621 / 418.4     # beta = 1/(KB_REF*T) per iu (src/MC_metropolis.cpp:175-204)
+
+
+def _water(nmol, seed, temp, tag):
+    L = (nmol / 0.0334) ** (1.0 / 3.0)
+    nside = int(math.ceil(nmol ** (1.0 / 3.0) - 1e-9))
+    s, _ = systems.water_box(seed=seed, nside=nside, L=L, nmol=nmol, temp=temp, tag=tag)
+    return s
+
+
+def _kT(s):
+    return 0.83144621 * s.temp
+
+
+def _tol(s, rel=1e-5):
+    kT = _kT(s)
+    return lambda de: rel * max(abs(de), kT)
+
+
+DE_CMP = 5e4   # iu; above it a move may have torn a molecule across the box edge
+
+
+def _compare(acc_b, de_b, acc_r, de_r, tol, tag, de_cmp=np.inf):
+    mism = np.nonzero(acc_b != acc_r)[0]
+    n_ok = len(acc_b) if len(mism) == 0 else int(mism[0])
+    for k in range(n_ok):
+        if abs(de_r[k]) < de_cmp:
+            assert abs(de_b[k] - de_r[k]) <= tol(de_r[k]), (tag, k, de_b[k], de_r[k])
+    if len(mism):
+        k = int(mism[0])
+        assert abs(de_b[k] - de_r[k]) <= tol(de_r[k]), (tag, "divergence outside the band", k, de_b[k], de_r[k])
+    return n_ok
+
+
+def _systems_mixed_set():
+    """Replicas of unequal size and content: a box smaller than twice the
+    cutoff, frozen atoms, and a chain whose atom moves have excluded
+    partners outside the moved set (bonded terms, 1-4 pairs)."""
+    out = [_water(90, 21, 290.0, 300), _water(300, 22, 305.0, 301), _water(700, 23, 320.0, 302)]
+    fz = out[1]
+    fz.frozen = np.zeros(fz.natom, dtype=np.int32)
+    fz.frozen[:90] = 1
+    mx, _ = systems.mixed_system(seed=3, nside=6, L=18.6, nbead=30)
+    out.append(mx)
+    return out
+
+
+def test_batch_replay_matches_oracle():
+    """std::mt19937 replay per replica: every replica's accept sequence and
+    dE follow the oracle's MC_metropolis::run restatement with list-free
+    moving-set energies (oracle mode 2)."""
+    reps = _systems_mixed_set()
+    seeds = [11, 12, 13, 14]
+    refs = [s.copy() for s in reps]
+    handles = [mcx.from_system(s) for s in reps]
+    b = mcx.Batch(handles, [s.moves for s in reps])
+    b.reset(0, mcx.MC_REPLAY, seeds)
+    n = 400
+    acc, de = b.moves(n)
+    st = b.state()
+    for r, (s, ref) in enumerate(zip(reps, refs)):
+        o = Oracle(ref)
+        o.update_list()
+        ro = o.mc_run(n, seeds[r], mode=2, list_every=50)
+        n_ok = _compare(acc[r], de[r], ro["acc"], ro["de"], _tol(s), "replica %d" % r)
+        assert n_ok == n, ("replica %d diverged at move %d" % (r, n_ok))
+        assert np.array_equal(st["trials"][r][: len(s.moves)], np.bincount(ro["mvtyp"], minlength=len(s.moves)))
+        x, y, z = handles[r].get_coords()
+        assert np.max(np.abs(x - ref.x)) < 1e-9 and np.max(np.abs(z - ref.z)) < 1e-9
+        e_fresh = handles[r].energy()[0]
+        assert abs(st["etot"][r] - e_fresh) <= 1e-6 * abs(e_fresh), (r, st["etot"][r], e_fresh)
+        assert abs(st["etot"][r] - ro["etot"]) <= 1e-6 * abs(ro["etot"]), (r, st["etot"][r], ro["etot"])
+
+
+def test_batch_matches_single_chain_loop():
+    """Philox streams: replica r of the batch and the single-chain device
+    loop on the same system with stream r make the same moves."""
+    reps = [_water(n, 30 + k, 280.0 + 20 * k, 310 + k) for k, n in enumerate((150, 400, 1000))]
+    seed, n = 99, 600
+    handles = [mcx.from_system(s.copy()) for s in reps]
+    b = mcx.Batch(handles, reps[0].moves)
+    b.reset(seed, mcx.MC_PHILOX)
+    acc, de = b.moves(n)
+    st = b.state()
+    for r, s in enumerate(reps):
+        ff = mcx.FField(s.copy())
+        ff.h.set_rng_stream(r)
+        res = ff.h.mc_run(s.moves, n, seed, rng_mode=mcx.MC_PHILOX, list_every=50)
+        n_ok = _compare(acc[r], de[r], res["acc"], res["de"], _tol(s, 1e-9), "replica %d" % r, DE_CMP)
+        assert n_ok == n, ("replica %d diverged at move %d" % (r, n_ok))
+        xb, yb, zb = handles[r].get_coords()
+        xs, ys, zs = ff.get_coords()
+        assert np.max(np.abs(xb - xs)) < 1e-12 and np.max(np.abs(zb - zs)) < 1e-12
+        assert abs(st["etot"][r] - res["etot"]) <= 1e-9 * abs(res["etot"])
+
+
+def test_batch_launch_shapes_agree():
+    reps = [_water(n, 40 + k, 300.0, 320 + k) for k, n in enumerate((120, 500, 2500))]
+    coords = [(s.x.copy(), s.y.copy(), s.z.copy()) for s in reps]
+    handles = [mcx.from_system(s) for s in reps]
+    b = mcx.Batch(handles, reps[0].moves)
+    base = None
+    for threads in (256, 512, 1024):
+        for unroll in (1, 2, 4):
+            for h, c in zip(handles, coords):
+                h.set_coords(*c)
+            b.set_config(threads, unroll)
+            b.reset(5, mcx.MC_PHILOX)
+            acc, de = b.moves(300)
+            if base is None:
+                base = (acc, de)
+                continue
+            assert np.array_equal(acc, base[0]), (threads, unroll)
+            assert np.max(np.abs(de - base[1]) / np.maximum(np.abs(base[1]), 1.0)) <= 1e-9, (threads, unroll)
+
+
+def _exchange_u(seed, k, rnd):
+    ctr = np.array([[k, rnd & 0xFFFFFFFF, (rnd >> 32) & 0xFFFFFFFF, 0x52455843]], dtype=np.uint32)
+    key = np.array([seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF], dtype=np.uint32)
+    w = philox.philox4x32_10(ctr, key)[0]
+    return ((int(w[0]) >> 5) * 67108864.0 + (int(w[1]) >> 6)) / 9007199254740992.0
+
+
+def _ladder_batch(nrep=6, nmol=80):
+    reps = [_water(nmol, 50 + k, 300.0, 330 + k) for k in range(nrep)]
+    temps = [300.0 + 1.5 * k for k in range(nrep)]
+    coords = [(s.x.copy(), s.y.copy(), s.z.copy()) for s in reps]
+    handles = [mcx.from_system(s) for s in reps]
+    b = mcx.Batch(handles, reps[0].moves)
+    b.set_ladder(temps)
+    b.set_sweep([25] * nrep)
+    return b, handles, coords, temps
+
+
+def test_replica_exchange_decisions():
+    """Each exchange round is recomputed on the host: slot pairs (k, k+1) of
+    the round's parity, min(1, exp((b_k - b_k+1)(E_k - E_k+1))) against the
+    round's Philox uniform."""
+    seed = 4242
+    b, handles, coords, temps = _ladder_batch()
+    b.reset(seed, mcx.MC_PHILOX)
+    beta = [1.0 / (KB_REF * t) for t in temps]
+    slot = np.arange(b.nrep)
+    n_acc = 0
+    for rnd in range(8):
+        out = b.run(2, exchange_every=2, use_graph=False)
+        assert out["exchange_attempts"] == len(range(rnd & 1, b.nrep - 1, 2))
+        st = b.state()
+        e = st["etot"]
+        rep_of_slot = np.argsort(slot)
+        expect = slot.copy()
+        for k in range(rnd & 1, b.nrep - 1, 2):
+            ra, rb = rep_of_slot[k], rep_of_slot[k + 1]
+            delta = (beta[k] - beta[k + 1]) * (e[ra] - e[rb])
+            if delta >= 0.0 or _exchange_u(seed, k, rnd) < math.exp(delta):
+                expect[ra], expect[rb] = k + 1, k
+                n_acc += 1
+        assert np.array_equal(st["slot"], expect), (rnd, st["slot"], expect)
+        slot = expect
+    att, acc = b.exchange_stats()
+    assert int(acc.sum()) == n_acc and int(att.sum()) == sum(len(range(r & 1, b.nrep - 1, 2)) for r in range(8))
+
+
+def test_graph_replay_matches_plain_launches():
+    b, handles, coords, temps = _ladder_batch()
+    results = []
+    for use_graph in (False, True):
+        for h, c in zip(handles, coords):
+            h.set_coords(*c)
+        b.reset(77, mcx.MC_PHILOX)
+        out = b.run(9, exchange_every=3, use_graph=use_graph)
+        st = b.state()
+        xs = [h.get_coords()[0] for h in handles]
+        results.append((out, st, xs))
+    (o1, s1, x1), (o2, s2, x2) = results
+    assert o1["accepted"] == o2["accepted"] and o1["exchange_accepted"] == o2["exchange_accepted"]
+    assert np.array_equal(s1["etot"], s2["etot"])
+    assert np.array_equal(s1["slot"], s2["slot"])
+    for a, c in zip(x1, x2):
+        assert np.array_equal(a, c)
+    for r, h in enumerate(handles):
+        e_fresh = h.energy()[0]
+        assert abs(s2["etot"][r] - e_fresh) <= 1e-6 * abs(e_fresh)
+
+
+def test_tuner_keeps_state():
+    reps = [_water(n, 60 + k, 300.0, 340 + k) for k, n in enumerate((100, 300, 900, 2000))]
+    handles = [mcx.from_system(s) for s in reps]
+    b = mcx.Batch(handles, reps[0].moves)
+    b.reset(3, mcx.MC_PHILOX)
+    b.run(1)
+    before = b.state()
+    xb = [h.get_coords() for h in handles]
+    threads, unroll, ms = b.tune()
+    assert threads in (256, 512, 1024) and unroll in (1, 2, 4) and ms > 0
+    after = b.state()
+    assert np.array_equal(before["etot"], after["etot"])
+    for h, c in zip(handles, xb):
+        assert np.array_equal(h.get_coords()[0], c[0])
+    out = b.run(2)
+    assert out["moves"] == 2 * sum(s.natom // 3 for s in reps)

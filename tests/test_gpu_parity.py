@@ -211,6 +211,26 @@ def test_mc_replay_water_residue_moves(which):
     assert abs(res["etot"] - ff.getE()) <= 1e-6 * abs(ff.getE())
 
 
+def test_mc_replay_small_box():
+    """Half the box (6.95 A) is shorter than the cutoff: every pair is in the
+    list and many moves change a pair's minimum image."""
+    s, _ = systems.water_box(seed=21, nside=5, L=(90 / 0.0334) ** (1 / 3), nmol=90, temp=290.0, tag=300)
+    o, ref = _pair(s)
+    o.update_list()
+    e0 = abs(o.getE()[0])        # the totals' 1e-6 bar applies to the starting energy too
+    ff = mcx.FField(s)
+    n = 2000
+    res = mcx.MC_metropolis(ff, nsteps=n, seed=11).run()
+    ro = o.mc_run(n, 11, mode=1)
+    kT = _kT(s)
+    n_ok = _compare_replay(res, ro, None, lambda de: 1e-5 * max(abs(de), kT))
+    assert n_ok == n
+    # running totals agree (each starts from its own full energy); the
+    # device's fresh total agrees with the FP64 one
+    assert abs(res["etot"] - ro["etot"]) <= 1e-6 * max(e0, abs(ro["etot"]))
+    assert abs(ff.getE() - Oracle(ref).getE()[0]) <= 1e-6 * max(e0, abs(ro["etot"]))
+
+
 def test_gradient_vs_fp64_and_fd():
     s, _ = systems.water_box(seed=3, nside=8, L=24.85)
     o, ref = _pair(s)

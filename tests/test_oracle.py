@@ -99,11 +99,15 @@ def test_reference_test_systems_bit_exact(oracle_lib, name):
         g["excl_flat_sha256"]
 
 
-@pytest.mark.parametrize("name", ["argon_fcc_seed1", "water512_seed3", "water4096_seed2"])
+SMALL_BOX = dict(seed=21, nside=5, L=(90 / 0.0334) ** (1 / 3), nmol=90, temp=290.0, tag=300)
+
+
+@pytest.mark.parametrize("name", ["argon_fcc_seed1", "water512_seed3", "water4096_seed2", "water90_small_box"])
 def test_synthetic_systems_bit_exact(oracle_lib, name):
     gen = {"argon_fcc_seed1": lambda: systems.argon_fcc(seed=1),
            "water512_seed3": lambda: systems.water_box(seed=3, nside=8, L=24.85),
-           "water4096_seed2": lambda: systems.water_box(seed=2)}[name]
+           "water4096_seed2": lambda: systems.water_box(seed=2),
+           "water90_small_box": lambda: systems.water_box(**SMALL_BOX)}[name]
     s, _ = gen()
     g = GOLD[name]
     o = O.Oracle(s)
@@ -115,11 +119,13 @@ def test_synthetic_systems_bit_exact(oracle_lib, name):
     assert O.Oracle(s2).getE().tolist() == g["energy_full"]
 
 
-@pytest.mark.parametrize("name", ["argon_fcc_seed1", "water512_seed3"])
+@pytest.mark.parametrize("name", ["argon_fcc_seed1", "water512_seed3", "water90_small_box"])
 def test_metropolis_run_matches_reference(oracle_lib, name):
-    """Accept counts, final dmax and final coordinates of MC_metropolis::run."""
+    """Accept counts, final dmax and final coordinates of MC_metropolis::run
+    (water90_small_box: half the box is shorter than the cutoff)."""
     gen = {"argon_fcc_seed1": lambda: systems.argon_fcc(seed=1),
-           "water512_seed3": lambda: systems.water_box(seed=3, nside=8, L=24.85)}[name]
+           "water512_seed3": lambda: systems.water_box(seed=3, nside=8, L=24.85),
+           "water90_small_box": lambda: systems.water_box(**SMALL_BOX)}[name]
     s, _ = gen()
     g = GOLD[name]["mc"]
     o = O.Oracle(s)
@@ -142,6 +148,65 @@ def test_moving_set_mode_equals_two_full_energies(oracle_lib):
     assert np.array_equal(ra["acc"], rb["acc"])
     # two full FP64 totals differ from the moving-set sum by their own rounding (~1e-12 of E_tot)
     assert np.max(np.abs(ra["de"] - rb["de"])) < 1e-11 * abs(ra["etot"])
+
+
+def _brute_moved_energy(s, first, last):
+    """Switched nonbonded energy of the pairs with an atom in [first, last]
+    (exclusions = intramolecular pairs, as water has them), numpy FP64."""
+    L = s.box
+    x = np.stack([s.x, s.y, s.z], 1)
+    C = 332.0716 * KCAL
+    e = 0.0
+    for i in range(first, last + 1):
+        d = x - x[i]
+        d -= L * np.where(d / L >= 0, np.trunc(d / L + 0.5), np.trunc(d / L - 0.5))
+        r2 = (d * d).sum(1)
+        m = (s.resid != s.resid[i]) & (r2 <= s.ctoff ** 2)
+        m &= ~((np.arange(s.natom) >= first) & (np.arange(s.natom) <= last) & (np.arange(s.natom) < i))
+        sw = np.where(r2 <= s.cton ** 2, 1.0,
+                      (s.ctoff ** 2 - r2) ** 2 * (s.ctoff ** 2 + 2 * r2 - 3 * s.cton ** 2) / (s.ctoff ** 2 - s.cton ** 2) ** 3)
+        r = np.sqrt(r2[m])
+        u = ((s.sig[i] + s.sig[m]) / r) ** 6
+        e += (sw[m] * (C * s.charge[i] * s.charge[m] / r + 4 * s.eps[i] * s.eps[m] * (u * u - u))).sum()
+    return e
+
+
+def test_exact_moving_set_energy(oracle_lib):
+    """The list-free moving-set energy: equal to the list-based one right
+    after a list update, and to a numpy brute force after a molecule jumps
+    farther than the list skin (when the stale list misses pairs)."""
+    s, _ = systems.water_box(seed=3, nside=8, L=24.85)
+    o = O.Oracle(s)
+    o.update_list()
+    bonded = {}
+    for k in (0, 77, 300, 511):
+        a, b = 3 * k, 3 * k + 2
+        bonded[k] = o.moving_set_energy(a, b) - _brute_moved_energy(s, a, b)
+        assert o.moving_set_energy_exact(a, b) == pytest.approx(o.moving_set_energy(a, b), rel=1e-12, abs=1e-9)
+        assert abs(bonded[k]) < 1e-6 * abs(o.moving_set_energy(a, b)) + 1e-3   # water's bonded terms at r0, theta0
+    for k in (0, 77, 300, 511):
+        a, b = 3 * k, 3 * k + 2
+        s.x[a:b + 1] += 4.0                    # jump by twice the skin; the list is now stale
+        s.x[:] = systems.wrap(s, s.x, s.y, s.z)[0]
+        exact = o.moving_set_energy_exact(a, b)
+        assert exact == pytest.approx(_brute_moved_energy(s, a, b) + bonded[k], rel=1e-10, abs=1e-6)
+        s.x[a:b + 1] -= 4.0
+        s.x[:] = systems.wrap(s, s.x, s.y, s.z)[0]
+
+
+def test_exact_mode_equals_list_mode_without_jumps(oracle_lib):
+    """TRN-only moves (0.3 A) never outrun the 2 A skin between list
+    updates: the list-free run makes the same moves as the list-based one."""
+    s, _ = systems.water_box(seed=3, nside=8, L=24.85)
+    s.moves = s.moves[:1]
+    a, b = s.copy(), s.copy()
+    oa, ob = O.Oracle(a), O.Oracle(b)
+    oa.update_list(); ob.update_list()
+    ra = oa.mc_run(1500, 9, mode=1)
+    rb = ob.mc_run(1500, 9, mode=2)
+    assert np.array_equal(ra["acc"], rb["acc"])
+    assert np.max(np.abs(ra["de"] - rb["de"])) < 1e-8
+    assert np.array_equal(a.x, b.x)
 
 
 def test_residue_range_last_atom_quirk(oracle_lib):
