@@ -8,6 +8,13 @@ residue TRN (0.3 A) and ROT (15 deg) moves.  A "step" is a window of
 and commit on the GPU), with the Verlet list rebuilt on the GPU every 50
 moves as the reference schedules it.  value = moves per second.
 
+Batched replicas (same run): 256 water boxes of 300-3,000 molecules
+(rho = 0.0334 / A^3, geometric temperature ladder 280-400 K), one persistent
+kernel launch per sweep with on-device replica exchange every 100 sweeps,
+replayed as a hipGraph, launch shape picked by the library's tuner; compared
+with the same replicas run one at a time through the single-chain device
+loop -> "batched".
+
 Also measured in the same run (HIP events on the library's own stream):
   * the O(N^2) pair kernel (full min-image LJ+Coulomb energy of the box):
     pair-evals/s and FP32 fraction of peak -> "roofline";
@@ -49,6 +56,42 @@ def cpu_reference_baseline(s, fp, nmoves, seed):
     return nmoves / dt, dt
 
 
+def batched_replicas(args):
+    """256-replica ladder: batched sweeps vs the same replicas looped one at a
+    time on the GPU (one sweep each through the single-chain device loop)."""
+    from mc_cpp_amd import mcx, systems
+    reps = systems.water_replicas(seed=4, nrep=args.replicas)
+    handles = [mcx.from_system(s) for s, _ in reps]
+    b = mcx.Batch(handles, reps[0][0].moves)
+    nrep = len(reps)
+    temps = [280.0 * (400.0 / 280.0) ** (k / max(nrep - 1, 1)) for k in range(nrep)]
+    b.set_ladder(temps)
+    b.reset(2024, mcx.MC_PHILOX)
+    threads, unroll, ms_tuned = b.tune()
+    b.run(args.batch_warmup, exchange_every=100, use_graph=True)
+    out = b.run(args.batch_sweeps, exchange_every=100, use_graph=True)
+    bmps = out["moves"] / out["seconds"]
+    tm = nm = 0.0
+    for (s, _), h in zip(reps, handles):
+        r = h.mc_run(s.moves, s.natom // 3, 99, rng_mode=mcx.MC_PHILOX, list_every=50, record=False)
+        tm += r["seconds"]
+        nm += s.natom // 3
+    pmps = nm / tm
+    natoms = sum(s.natom for s, _ in reps)
+    return {"workload": "%d water boxes, %d-%d molecules (%d atoms), T ladder 280-400 K, exchange every 100 sweeps, "
+                        "residue TRN 0.3 A + ROT 15 deg, sweep = one move per molecule"
+                        % (nrep, min(s.natom for s, _ in reps) // 3, max(s.natom for s, _ in reps) // 3, natoms),
+            "moves_per_s": bmps, "sweeps": args.batch_sweeps, "moves": out["moves"], "seconds": out["seconds"],
+            "ms_per_sweep": out["seconds"] / args.batch_sweeps * 1e3,
+            "pair_evals_per_s": 2.0 * out["pair_evals"] / out["seconds"],
+            "acceptance": out["accepted"] / out["moves"],
+            "exchange_attempts": out["exchange_attempts"], "exchange_accepted": out["exchange_accepted"],
+            "launch_shape": {"threads": threads, "tiles_in_flight": unroll, "tuned_ms_per_sweep": ms_tuned},
+            "plain_loop_moves_per_s": pmps,
+            "plain_loop": "each replica one sweep through the single-chain device loop (mcx_mc_run, list every 50)",
+            "speedup_vs_plain": bmps / pmps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -57,6 +100,10 @@ def main():
     ap.add_argument("--moves", type=int, default=20000, help="Metropolis moves per step")
     ap.add_argument("--cpu-moves", type=int, default=24, help="moves of the reference CPU loop to time")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--replicas", type=int, default=256)
+    ap.add_argument("--batch-sweeps", type=int, default=100)
+    ap.add_argument("--batch-warmup", type=int, default=1)
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the Metropolis chain is sequential and this build has no "
@@ -110,6 +157,8 @@ def main():
     list_bytes = 24 * n + 4 * (n + 1) + 4 * npairs
     list_gbs = list_bytes / (ms_list * 1e-3) / 1e9
 
+    batch = None if args.no_batch else batched_replicas(args)
+
     cpu = None
     if not args.no_cpu:
         res = cpu_reference_baseline(s_cpu, fp, args.cpu_moves, 7)
@@ -148,6 +197,7 @@ def main():
         "list_build": {"bound": "hbm", "achieved": list_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                        "frac": list_gbs / PEAK_HBM_GBS, "bytes_per_launch": list_bytes, "ms_per_launch": ms_list,
                        "pairs": npairs},
+        "batched": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -206,6 +206,7 @@ typedef struct mcx_batch_stats {
     long long exchange_accepted;
     long long sweeps;              /* sweeps since the last reset */
     double seconds;                /* wall time of the call's device work */
+    long long pair_evals;          /* in-cutoff pair changes evaluated (old and new energy each) */
 } mcx_batch_stats;
 
 #define MCX_MAX_MOVE_TYPES 8

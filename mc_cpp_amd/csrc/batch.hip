@@ -112,7 +112,7 @@ __device__ __forceinline__ double wave_max0(double v)
 
 // Pair change of moved atom a (old image p, displacement g, parameters qa)
 // against atom j, which does not move.
-__device__ __forceinline__ void pair_moved_fixed(const DevSys &s, const int4 &p, double gx, double gy, double gz,
+__device__ __forceinline__ bool pair_moved_fixed(const DevSys &s, const int4 &p, double gx, double gy, double gz,
                                                  const double4 &qa, const int4 &pj, const double4 &P, bool check,
                                                  double &dec, double &dev)
 {
@@ -123,7 +123,9 @@ __device__ __forceinline__ void pair_moved_fixed(const DevSys &s, const int4 &p,
     if (mcx_pair_change(s, ox, oy, oz, -gx, -gy, -gz, qa.x, qa.y, qa.z, P.x, P.y, P.z, check, c, v)) {
         dec += c;
         dev += v;
+        return true;
     }
+    return false;
 }
 
 // Nonbonded part of dE from wavefronts 1..NT/64-1: every atom outside the
@@ -134,7 +136,7 @@ __device__ __forceinline__ void pair_moved_fixed(const DevSys &s, const int4 &p,
 // share of dE in iu.
 template <int NT, int U, bool IMG, int NM, typename QT>
 __device__ __forceinline__ double scan_pairs(const DevSys &s, const BMove &m, const ImgFetch<IMG> &f,
-                                             const double4 *ptab, QT *queue, bool frozen_any)
+                                             const double4 *ptab, QT *queue, bool frozen_any, unsigned &npair)
 {
     constexpr int NSW = NT / 64 - 1;
     constexpr int NR = NM > 0 ? NM : 1;
@@ -165,7 +167,7 @@ __device__ __forceinline__ double scan_pairs(const DevSys &s, const BMove &m, co
         const bool fj = (pj.w >> 16) != 0;
         auto one = [&](const int4 &p, double gx, double gy, double gz, const double4 &qq) {
             if (frozen_any && fj && (p.w >> 16)) return;
-            pair_moved_fixed(s, p, gx, gy, gz, qq, pj, P, check, dec, dev);
+            npair += pair_moved_fixed(s, p, gx, gy, gz, qq, pj, P, check, dec, dev);
         };
         if (NM > 0) {
 #pragma unroll
@@ -268,6 +270,7 @@ __global__ __launch_bounds__(NT) void k_batch_sweep(BatchArgs a)
         for (int k = tid; k < kMTN; k += NT) mt[k] = R.rng[k];
     __syncthreads();
     const int nmoves = a.nmoves > 0 ? a.nmoves : R.sweep_len;
+    unsigned npair = 0;     // this lane's in-cutoff pair evaluations (old and new) of the scan
 
     for (int mi = 0; mi < nmoves; mi++) {
         // ---- wavefront 0: the move (src/MC_metropolis.cpp:85-116;
@@ -381,7 +384,8 @@ __global__ __launch_bounds__(NT) void k_batch_sweep(BatchArgs a)
         double part = 0.0;
         if (w0) {
             double dec = 0.0, dev = 0.0, deb = 0.0;
-            const CoordFetch fo{s.x, s.y, s.z, nullptr, nullptr, nullptr, 0, -1};
+            // moved atoms' old and trial coordinates from LDS, others from memory
+            const CoordFetch fo{s.x, s.y, s.z, m.ox, m.oy, m.oz, first, last};
             const CoordFetch fn{s.x, s.y, s.z, m.tx, m.ty, m.tz, first, last};
             auto term = [&](const int4 &id, int kk, int ty_, const double4 &par) {
                 double e0, c0, v0, e1, c1, v1;
@@ -426,9 +430,9 @@ __global__ __launch_bounds__(NT) void k_batch_sweep(BatchArgs a)
             part = MCX_CHGCHARMM * MCX_KCALTOIU * dec + 4.0 * dev + deb;
         } else {
             const int n_m = m.nm;
-            if (n_m == 3) part = scan_pairs<NT, U, IMG, 3, QT>(s, m, f, ptab, queue, frozen_any);
-            else if (n_m == 1) part = scan_pairs<NT, U, IMG, 1, QT>(s, m, f, ptab, queue, frozen_any);
-            else part = scan_pairs<NT, U, IMG, 0, QT>(s, m, f, ptab, queue, frozen_any);
+            if (n_m == 3) part = scan_pairs<NT, U, IMG, 3, QT>(s, m, f, ptab, queue, frozen_any, npair);
+            else if (n_m == 1) part = scan_pairs<NT, U, IMG, 1, QT>(s, m, f, ptab, queue, frozen_any, npair);
+            else part = scan_pairs<NT, U, IMG, 0, QT>(s, m, f, ptab, queue, frozen_any, npair);
         }
         part = wave_total(part);
         if (lane == 0) m.red[tid >> 6] = part;
@@ -469,6 +473,11 @@ __global__ __launch_bounds__(NT) void k_batch_sweep(BatchArgs a)
                 if (IMG) img[i] = make_int4(p.x, p.y, p.z, tw);
             }
         }
+    }
+    // pair evaluations of the launch: one atomic per wavefront
+    {
+        const double tw = wave_total((double)npair);
+        if (lane == 0 && tw > 0.0) atomicAdd(R.npair, (unsigned long long)tw);
     }
     __syncthreads();
     if (w0) {

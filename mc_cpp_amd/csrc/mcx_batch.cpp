@@ -31,7 +31,7 @@ struct mcx_batch {
     DBuf<double> dtemps, etot, dmax;
     DBuf<uint32_t> rng;
     DBuf<long long> trials, accepted, step, xatt, xacc;
-    DBuf<unsigned long long> xcount;
+    DBuf<unsigned long long> xcount, npair;
     std::vector<size_t> atype_off;   // offset of replica r's atom types in the pool
     int max_n = 0;
     size_t lds_limit = 0;
@@ -98,6 +98,7 @@ struct mcx_batch {
         drep.release(); order.release(); slot_of_rep.release(); rep_of_slot.release(); atype.release();
         acc_tmp.release(); ptab.release(); dtemps.release(); etot.release(); dmax.release(); rng.release();
         trials.release(); accepted.release(); step.release(); xatt.release(); xacc.release(); xcount.release();
+        npair.release();
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -193,6 +194,7 @@ void build_descriptors(mcx_batch *b)
         R.accepted = b->accepted.p + (size_t)r * kMaxMoveTypes;
         R.acc_tmp = b->acc_tmp.p + (size_t)r * kMaxMoveTypes;
         R.step = b->step.p + r;
+        R.npair = b->npair.p + r;
     }
     std::vector<int> ord(b->nrep);
     for (int r = 0; r < b->nrep; r++) ord[r] = r;
@@ -256,6 +258,8 @@ void reset(mcx_batch *b, uint64_t seed, int rng_mode, const uint64_t *seeds)
     b->xatt.upload(zx.data(), zx.size(), b->stream);
     b->xacc.upload(zx.data(), zx.size(), b->stream);
     b->xcount.upload(&zc, 1, b->stream);
+    std::vector<unsigned long long> zp(R, 0);
+    b->npair.upload(zp.data(), R, b->stream);
     HIPCHK(hipStreamSynchronize(b->stream));
     b->sweeps = 0;
 }
@@ -269,6 +273,15 @@ void finish(mcx_batch *b)
         sync_host_coords(h);
         h->list_valid = false;
     }
+}
+
+unsigned long long sum_pairs(mcx_batch *b)
+{
+    std::vector<unsigned long long> a(b->nrep);
+    HIPCHK(hipMemcpy(a.data(), b->npair.p, a.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    unsigned long long s = 0;
+    for (unsigned long long v : a) s += v;
+    return s;
 }
 
 long long sum_accepted(mcx_batch *b)
@@ -408,6 +421,7 @@ int mcx_batch_create(mcx_ctx *const *reps, int nrep, const mcx_move_type *types,
         b->accepted.alloc(m); b->acc_tmp.alloc(m); b->step.alloc(R);
         b->slot_of_rep.alloc(R); b->rep_of_slot.alloc(R);
         b->xatt.alloc(std::max(nrep - 1, 1)); b->xacc.alloc(std::max(nrep - 1, 1)); b->xcount.alloc(1);
+        b->npair.alloc(R);
         b->dtemps.upload(b->temps.data(), R, b->stream);
         reset(b.get(), 0, MCX_MC_PHILOX, nullptr);
         b->prepare();
@@ -473,6 +487,7 @@ int mcx_batch_run(mcx_batch *b, int nsweeps, int exchange_every, int use_graph, 
     return bguard(b, [&] {
         if (nsweeps < 0 || exchange_every < 0) throw mcx_error(MCX_ERR_ARG, "negative sweep count");
         const long long acc0 = stats ? sum_accepted(b) : 0;
+        const unsigned long long np0 = stats ? sum_pairs(b) : 0;
         long long xa0 = 0, xc0 = 0;
         if (stats) exchange_totals(b, xa0, xc0);
         HIPCHK(hipStreamSynchronize(b->stream));
@@ -547,6 +562,7 @@ int mcx_batch_run(mcx_batch *b, int nsweeps, int exchange_every, int use_graph, 
             stats->exchange_accepted = xc - xc0;
             stats->sweeps = b->sweeps;
             stats->seconds = secs;
+            stats->pair_evals = (long long)(sum_pairs(b) - np0);
         }
     });
 }

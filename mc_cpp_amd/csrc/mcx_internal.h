@@ -191,6 +191,7 @@ struct BatchRep {
     long long *trials, *accepted;   // [kMaxMoveTypes]
     int *acc_tmp;            // [kMaxMoveTypes]
     long long *step;         // moves done since the chain was (re)started
+    unsigned long long *npair;   // in-cutoff pair changes evaluated (each: old and new energy)
 };
 struct BatchArgs {
     const BatchRep *reps;

@@ -51,7 +51,8 @@ class MCStatsC(C.Structure):
 
 class BatchStatsC(C.Structure):
     _fields_ = [("moves", C.c_longlong), ("accepted", C.c_longlong), ("exchange_attempts", C.c_longlong),
-                ("exchange_accepted", C.c_longlong), ("sweeps", C.c_longlong), ("seconds", C.c_double)]
+                ("exchange_accepted", C.c_longlong), ("sweeps", C.c_longlong), ("seconds", C.c_double),
+                ("pair_evals", C.c_longlong)]
 
 
 _llp = C.POINTER(C.c_longlong)
@@ -427,7 +428,8 @@ class Batch:
         st = BatchStatsC()
         self._chk(self.L.mcx_batch_run(self.b, int(nsweeps), int(exchange_every), int(bool(use_graph)), C.byref(st)))
         return dict(moves=st.moves, accepted=st.accepted, exchange_attempts=st.exchange_attempts,
-                    exchange_accepted=st.exchange_accepted, sweeps=st.sweeps, seconds=st.seconds)
+                    exchange_accepted=st.exchange_accepted, sweeps=st.sweeps, seconds=st.seconds,
+                    pair_evals=st.pair_evals)
 
     def moves(self, nmoves, record=True):
         acc = np.zeros((self.nrep, nmoves), np.uint8) if record else None
