@@ -278,6 +278,7 @@ __global__ __launch_bounds__(NT) void k_batch_sweep(BatchArgs a)
         // (MOVE_TRN::tr_a_b / MOVE_ROT::apply_rotation,
         // include/Move.hpp:42-50,120-143), with no barrier in between
         int tp = 0, first = 0, last = 0, nm = 0, ra0 = 0, ra1 = 0, pa0 = 0, pa1 = 0, inst = 0;
+        bool rigid = false;
         if (w0) {
             tp = (int)((double)tab.ntypes * dr.next(lane));
             inst = (int)((double)tab.ninst[tp] * dr.next(lane));
@@ -340,12 +341,15 @@ __global__ __launch_bounds__(NT) void k_batch_sweep(BatchArgs a)
             const int p0x = __builtin_amdgcn_readlane(pl.x, 0), p0y = __builtin_amdgcn_readlane(pl.y, 0);
             const int p0z = __builtin_amdgcn_readlane(pl.z, 0);
             double g2 = 0.0, rad2 = 0.0;
+            bool far = false;
             if (lane < nm) {
                 double ux, uy, uz;
                 if (type == MCX_MV_TRN_DEV) {
                     ux = x + rv[0]; uy = y + rv[1]; uz = z + rv[2];
                 } else {
                     const double px = x - piv[0], py = y - piv[1], pz = z - piv[2];
+                    const double lq = 0.25 * fmin(s.L[0], fmin(s.L[1], s.L[2]));
+                    far = s.pbtype != 0 && px * px + py * py + pz * pz >= lq * lq;
                     ux = piv[0] + Rm[0][0] * px + Rm[1][0] * py + Rm[2][0] * pz;
                     uy = piv[1] + Rm[0][1] * px + Rm[1][1] * py + Rm[2][1] * pz;
                     uz = piv[2] + Rm[0][2] * px + Rm[1][2] * py + Rm[2][2] * pz;
@@ -362,6 +366,14 @@ __global__ __launch_bounds__(NT) void k_batch_sweep(BatchArgs a)
                 rad2 = rx * rx + ry * ry + rz * rz;
             }
             const double gm = sqrt(wave_max0(g2)), rm = sqrt(wave_max0(rad2));
+            // A translation, or a rotation whose atoms all lie within a
+            // quarter box of the pivot, moves the set rigidly: distances,
+            // angles and torsions inside it change by rounding only
+            // (~1e-15 relative), so the terms among its own atoms are
+            // skipped.  A rotation of a set the box edge splits (the pivot
+            // is applied to wrapped coordinates, include/Move.hpp:120-143)
+            // is not rigid and keeps them.
+            rigid = __ballot(far) == 0;
             if (lane == 0) {
                 // |x_j - x_a| >= |x_j - x_0| - rm and a moves by at most gm, so
                 // a candidate farther than rc + rm + gm from atom 0's old
@@ -395,10 +407,16 @@ __global__ __launch_bounds__(NT) void k_batch_sweep(BatchArgs a)
                 dec += c1 - c0;
                 dev += v1 - v0;
             };
+            auto inside = [&](const int4 &id) {
+                return id.x >= first && id.x <= last && id.y >= first && id.y <= last &&
+                       (id.z < 0 || (id.z >= first && id.z <= last)) && (id.w < 0 || (id.w >= first && id.w <= last));
+            };
             for (int e = ra0 + lane; e < ra1; e += 64) {
                 const int k = e - ra0;
-                if (k < kBRec) term(m.rid[k], m.rkk[k], m.rty[k], m.rpar[k]);
-                else term(mv.rec_ids[e], mv.rec_kk[e], mv.rec_type[e], mv.rec_par[e]);
+                const int4 id = k < kBRec ? m.rid[k] : mv.rec_ids[e];
+                if (rigid && inside(id)) continue;
+                if (k < kBRec) term(id, m.rkk[k], m.rty[k], m.rpar[k]);
+                else term(id, mv.rec_kk[e], mv.rec_type[e], mv.rec_par[e]);
             }
             // pair corrections: excluded partners outside the moved range,
             // removed with the scan's own values, and pairs of two moved atoms
@@ -413,7 +431,7 @@ __global__ __launch_bounds__(NT) void k_batch_sweep(BatchArgs a)
                                      c, v);
                     dec -= c;
                     dev -= v;
-                } else {
+                } else if (!rigid) {
                     const int ib = pe.y;
                     const int4 pj = m.ip[ib];
                     const double ox = (double)mcx_wdiff(pj.x, p.x) * s.unit[0];
