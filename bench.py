@@ -15,6 +15,11 @@ replayed as a hipGraph, launch shape picked by the library's tuner; compared
 with the same replicas run one at a time through the single-chain device
 loop -> "batched".
 
+Force-bias (same run): residue force-bias moves (A = 0.01 A^2) against
+plain residue translations on the 12,288-atom box, moves/s and acceptance;
+the adjoint kernel's time on that box and a 64-configuration batched
+adjoint call -> "force_bias".
+
 Also measured in the same run (HIP events on the library's own stream):
   * the O(N^2) pair kernel (full min-image LJ+Coulomb energy of the box):
     pair-evals/s and FP32 fraction of peak -> "roofline";
@@ -92,6 +97,43 @@ def batched_replicas(args):
             "speedup_vs_plain": bmps / pmps}
 
 
+def force_bias_vs_translation(args):
+    """Force-bias residue moves (dx = beta A F + sqrt(2A) xi, A = 0.01 A^2,
+    proposal-density ratio in the acceptance) against plain residue
+    translations (0.3 A) on the 12,288-atom water box, plus the adjoint
+    kernel alone and a 64-system batched adjoint call."""
+    from mc_cpp_amd import mcx, systems
+    out = {}
+    for name, mt in (("force_bias", systems.MoveType(systems.MV_FBIAS, systems.MODE_RESIDUE, 0.01, 50.0, 0)),
+                     ("translation", systems.MoveType(systems.MV_TRN, systems.MODE_RESIDUE, 0.3, 50.0, 0))):
+        s, _ = systems.water_box(seed=2)
+        ff = mcx.FField(s)
+        ff.h.mc_run([mt], 2000, 5, rng_mode=mcx.MC_PHILOX, list_every=50, record=False)   # warm-up
+        r = ff.h.mc_run([mt], args.fb_moves, 6, rng_mode=mcx.MC_PHILOX, list_every=50, record=False)
+        out[name] = {"moves_per_s": args.fb_moves / r["seconds"],
+                     "acceptance": float(r["accepted"][0]) / args.fb_moves, "moves": args.fb_moves}
+    out["force_bias_over_translation"] = out["force_bias"]["moves_per_s"] / out["translation"]["moves_per_s"]
+    # adjoint kernel: one launch over the 12,288-atom box (HIP events)
+    s, _ = systems.water_box(seed=2)
+    h = mcx.FField(s).h
+    ms = h.time_kernel(4, 50)
+    npairs = h.update_list()
+    out["adjoint_kernel_ms"] = ms
+    out["adjoint_pair_visits_per_s"] = 2.0 * npairs / (ms * 1e-3)
+    # 64 configurations in one batched adjoint call (includes the copy of
+    # six gradients per atom back to the host)
+    boxes = [systems.water_box(seed=100 + k)[0] for k in range(64)]
+    hs = [mcx.FField(b).h for b in boxes]
+    mcx.grad_batch(hs)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        mcx.grad_batch(hs)
+    out["batched_adjoint"] = {"systems": 64, "atoms_each": boxes[0].natom,
+                              "ms_per_call": (time.perf_counter() - t0) / 3 * 1e3,
+                              "note": "wall time of mcx_nonbonded_grad_batch incl. the device-to-host copy"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +146,8 @@ def main():
     ap.add_argument("--batch-sweeps", type=int, default=100)
     ap.add_argument("--batch-warmup", type=int, default=1)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--fb-moves", type=int, default=20000, help="moves per force-bias / translation run")
+    ap.add_argument("--no-fbias", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the Metropolis chain is sequential and this build has no "
@@ -158,6 +202,7 @@ def main():
     list_gbs = list_bytes / (ms_list * 1e-3) / 1e9
 
     batch = None if args.no_batch else batched_replicas(args)
+    fbias = None if args.no_fbias else force_bias_vs_translation(args)
 
     cpu = None
     if not args.no_cpu:
@@ -198,6 +243,7 @@ def main():
                        "frac": list_gbs / PEAK_HBM_GBS, "bytes_per_launch": list_bytes, "ms_per_launch": ms_list,
                        "pairs": npairs},
         "batched": batch,
+        "force_bias": fbias,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -49,6 +49,7 @@ extern "C" {
 
 #define MCX_MV_TRN 1
 #define MCX_MV_ROT 2
+#define MCX_MV_FBIAS 3   /* force-bias (smart MC) translation; not in the reference */
 #define MCX_MODE_RESIDUE 1
 #define MCX_MODE_ALL     2
 #define MCX_MODE_ATOM    4
@@ -130,9 +131,9 @@ int mcx_delta_energy(mcx_ctx *h, int first, int last, const double *nx, const do
 /* One Monte Carlo move type, as List_Moves::addNewMoveType registers it
  * (src/List_Moves.cpp:170-211,257-457).  sele: 0/1 per atom or NULL = all. */
 typedef struct mcx_move_type {
-    int type;          /* MCX_MV_TRN / MCX_MV_ROT */
+    int type;          /* MCX_MV_TRN / MCX_MV_ROT / MCX_MV_FBIAS */
     int mode;          /* MCX_MODE_RESIDUE / MCX_MODE_ATOM / MCX_MODE_ALL */
-    double dmax;       /* A for TRN, degrees for ROT */
+    double dmax;       /* A for TRN, degrees for ROT, the step A (A^2) for FBIAS */
     double target;     /* acceptance target in % */
     int each;          /* dmax update period in steps, 0 = never */
     const int *sele;
@@ -160,6 +161,34 @@ int mcx_mc_run(mcx_ctx *h, const mcx_move_type *types, int ntypes, int nsteps, u
                int rng_mode, int list_every, mcx_mc_stats *stats, unsigned char *acc_out,
                double *de_out, int *mvtyp_out, int *mvinst_out);
 
+/* Per-step records of mcx_mc_run_traced; every pointer may be NULL. */
+typedef struct mcx_mc_trace {
+    unsigned char *acc;    /* [nsteps] accepted flag */
+    double *de;            /* [nsteps] dE (iu) */
+    int *mvtyp;            /* [nsteps] move type index */
+    int *mvinst;           /* [nsteps] move instance */
+    double *disp;          /* [3*nsteps] translation (A) of TRN and FBIAS moves, 0 for ROT */
+    double *lnq;           /* [nsteps] ln q(new->old)/q(old->new) of FBIAS moves, 0 otherwise */
+} mcx_mc_trace;
+
+/* mcx_mc_run with the trace records above; mcx_mc_run is this call with
+ * disp = lnq = NULL.  Not in the reference (it adds the force-bias records).
+ *
+ * MCX_MV_FBIAS moves (force-bias / smart Monte Carlo, Rossky, Doll and
+ * Friedman, J. Chem. Phys. 69, 4628 (1978); not in the reference) translate
+ * the move's atom set by
+ *     dx = beta A F + sqrt(2A) xi,   xi ~ N(0, I),
+ * F the net nonbonded force on the set from atoms outside it in kcal/mol/A
+ * (the list pairs of the set's atoms; bonded and 1-4 terms are not part of
+ * F), A = dmax (A^2, adapted like a TRN step), beta = 1/(R T) in mol/kcal,
+ * and accept with min(1, exp(-beta dE) q(new->old) / q(old->new)), beta dE
+ * formed as for TRN/ROT moves (dE in iu, as the reference's test does), ln q
+ * ratio = (|dx - beta A F_old|^2 - |-dx - beta A F_new|^2) / (4A).  Draws per move: move type, instance, four uniforms for three
+ * normals (Box-Muller on 1 - u), and the acceptance uniform only when the
+ * exponent is negative.  Batches (mcx_batch_create) refuse FBIAS types. */
+int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int nsteps, uint64_t seed,
+                      int rng_mode, int list_every, mcx_mc_stats *stats, const mcx_mc_trace *trace);
+
 /* Philox stream id used by mcx_mc_run in MCX_MC_PHILOX mode (default 0).
  * Replica r of a batch draws from stream r, so a handle set to stream r
  * replays that replica's proposals.  Not in the reference. */
@@ -171,13 +200,26 @@ int mcx_set_rng_stream(mcx_ctx *h, uint32_t stream);
 int mcx_nonbonded_grad(mcx_ctx *h, double *gx, double *gy, double *gz,
                        double *geps, double *gsig, double *gq);
 
+/* The same gradients with the parameter part summed per atom type (FP64,
+ * fixed order): type[i] in [0, ntypes) for every atom; gx/gy/gz per atom
+ * (n entries), geps/gsig/gq per type (ntypes entries).  Not in the reference. */
+int mcx_nonbonded_grad_types(mcx_ctx *h, int ntypes, const int *type, double *gx, double *gy, double *gz,
+                             double *geps, double *gsig, double *gq);
+
+/* Batched adjoint: the gradients of nsys systems (SWITCH mode, one device)
+ * in one launch.  g6[k] receives system k's 6*n values: gx, gy, gz, geps,
+ * gsig, gq back to back.  Errors are reported on hs[0].  Not in the
+ * reference. */
+int mcx_nonbonded_grad_batch(mcx_ctx *const *hs, int nsys, double *const *g6);
+
 /* Device pointer access for zero-copy interop (PyTorch tensors, HIP code):
  * which = 0 x, 1 y, 2 z (double[N] master coordinates). */
 int mcx_device_coords(mcx_ctx *h, int which, void **ptr);
 
 /* Timing probe used by the benchmark: runs `reps` back-to-back launches of
  * kernel `which` (0 full O(N^2) energy, 1 switched list energy, 2 list
- * build, 3 fused bonded) on the handle's stream and returns the mean
+ * build, 3 fused bonded, 4 nonbonded adjoint) on the handle's stream and
+ * returns the mean
  * device time per launch from HIP events. */
 int mcx_time_kernel(mcx_ctx *h, int which, int reps, double *ms_per_launch);
 

@@ -5,8 +5,10 @@
 // C = 332.0716*418.4, S the switching polynomial.  One wavefront per atom i
 // walks i's FULL list row (both directions), so every per-atom sum --
 // dE/dx_i, dE/de_i, dE/ds_i, dE/dq_i -- is owned by one wavefront: no
-// atomics, deterministic FP64 results.  Pair arithmetic FP32; S, dS/dr^2
-// and all sums FP64.
+// atomics, deterministic FP64 results.  LJ pair arithmetic FP32; Coulomb,
+// S, dS/dr^2 and all sums FP64.  A batched launch covers many systems
+// (blockIdx.y = system), and a second kernel reduces the per-atom parameter
+// gradients to per-type sums in FP64.
 #include "terms.h"
 
 namespace {
@@ -20,11 +22,9 @@ __device__ __forceinline__ double wsum(double v)
     return v;
 }
 
-__global__ __launch_bounds__(64 * kGW) void k_grad(DevSys s, double *__restrict__ g)
+// Gradient row of atom i; lane 0 writes the six components into g.
+__device__ __forceinline__ void grad_row(const DevSys &s, int i, int lane, double *__restrict__ g)
 {
-    const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * kGW + (threadIdx.x >> 6);
-    if (i >= s.n) return;
     const double C = MCX_CHGCHARMM * MCX_KCALTOIU;
     const float qi = __int_as_float(s.ipos[i].w);
     const float2 ei = s.es[i];
@@ -78,9 +78,55 @@ __global__ __launch_bounds__(64 * kGW) void k_grad(DevSys s, double *__restrict_
     }
 }
 
+__global__ __launch_bounds__(64 * kGW) void k_grad(DevSys s, double *__restrict__ g)
+{
+    const int i = blockIdx.x * kGW + (threadIdx.x >> 6);
+    if (i < s.n) grad_row(s, i, threadIdx.x & 63, g);
+}
+
+// One launch for many systems: blockIdx.y selects the system.
+__global__ __launch_bounds__(64 * kGW) void k_grad_batch(const DevSys *__restrict__ sys, double *const *__restrict__ g)
+{
+    const DevSys &s = sys[blockIdx.y];
+    const int i = blockIdx.x * kGW + (threadIdx.x >> 6);
+    if (i < s.n) grad_row(s, i, threadIdx.x & 63, g[blockIdx.y]);
+}
+
+// Per-type sums of the per-atom parameter gradients (components 3..5 of
+// g6), one block per type, fixed summation order.
+__global__ __launch_bounds__(256) void k_type_sums(const double *__restrict__ g6, int n, const int *__restrict__ type,
+                                                   double *__restrict__ out)
+{
+    __shared__ double red[3][4];
+    const int t = blockIdx.x;
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < n; i += 256)
+        if (type[i] == t)
+            for (int c = 0; c < 3; c++) a[c] += g6[(size_t)(3 + c) * n + i];
+    for (int c = 0; c < 3; c++) {
+        const double v = wsum(a[c]);
+        if ((threadIdx.x & 63) == 0) red[c][threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int c = threadIdx.x;
+        out[3 * t + c] = red[c][0] + red[c][1] + red[c][2] + red[c][3];
+    }
+}
+
 }  // namespace
 
 void launch_nonbonded_grad(const DevSys &s, double *g6, hipStream_t st)
 {
     hipLaunchKernelGGL(k_grad, dim3((s.n + kGW - 1) / kGW), dim3(64 * kGW), 0, st, s, g6);
+}
+
+void launch_nonbonded_grad_batch(const DevSys *sys, double *const *g6, int nsys, int nmax, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_grad_batch, dim3((nmax + kGW - 1) / kGW, nsys), dim3(64 * kGW), 0, st, sys, g6);
+}
+
+void launch_type_sums(const double *g6, int n, const int *type, int ntypes, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_type_sums, dim3(ntypes), dim3(256), 0, st, g6, n, type, out);
 }

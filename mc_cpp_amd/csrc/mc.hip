@@ -5,10 +5,11 @@
 // workgroup evaluates only the terms that touch the moved atoms:
 //   * nonbonded pairs from the moved atoms' full list rows (SWITCH) or all
 //     atoms (FULL): each pair's old and new energy from the exact
-//     fixed-point displacement and its change, with the distance chain and
-//     Coulomb in FP64 (so the per-pair difference is exact to ~1e-15
-//     relative), LJ in FP32, the switching polynomial and all sums in FP64;
+//     fixed-point displacement and its change, in FP64 (mcx_pair_change);
 //   * bonded and 1-4 terms through per-move term records, FP64.
+// Force-bias translations (not in the reference) add two force passes over
+// the same pairs: the net force on the moved set before the move (for the
+// drift) and after it (for the reverse proposal density).
 // The move itself (src/MC_metropolis.cpp:84-119; include/Move.hpp:42-183),
 // the draws (src/MC.cpp:127-156: std::mt19937 + generate_canonical, or a
 // Philox4x32-10 stream), the test of apply_criterion (:175-204) and the
@@ -42,15 +43,27 @@ struct MoveLDS {
     int4 rid[kMaxRec];
     int rkk[kMaxRec], rty[kMaxRec];
     double4 rpar[kMaxRec];
-    double red[3][kMCW];
+    double red[6][kMCW];
     double de;
+    double xi[3];                // force bias: the Gaussian draw
+    double fold[3], fnew[3];     // force bias: net force on the set before / after (iu/A)
 };
+
+__device__ __forceinline__ bool excluded(const DevSys &s, int i, int j)
+{
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    for (int k = s.excl_off[lo]; k < s.excl_off[lo + 1]; k++)
+        if (s.excl_idx[k] == hi) return true;
+    return false;
+}
 
 // dE contribution of pair (i, j) with i moved (index a) and j either moved
 // (index b >= 0, new coordinates tx/ty/tz[b]) or fixed (b < 0).
-// Adds q_i-scaled elec_raw and e_i-scaled vdw_raw.
-__device__ void pair_delta(const DevSys &s, const MoveLDS &m, int i, int a, int j, int b, const int4 pj,
-                           const double4 pq, double &dec, double &dev)
+// Adds q_i-scaled elec_raw and e_i-scaled vdw_raw; with `force` set and j
+// fixed, also the force on i at its new position to fnew.
+__device__ __forceinline__ void pair_delta(const DevSys &s, const MoveLDS &m, int i, int a, int j, int b,
+                                           const int4 pj, const double4 pq, double &dec, double &dev, bool force,
+                                           double (&fnew)[3])
 {
     const int4 pi = m.ip[a];
     double o[3];                                  // old displacement x_j - x_i
@@ -69,15 +82,58 @@ __device__ void pair_delta(const DevSys &s, const MoveLDS &m, int i, int a, int 
         dec += c;
         dev += v;
     }
+    if (force && b < 0) {
+        double dn[3] = {o[0] + g[0], o[1] + g[1], o[2] + g[2]};
+        if (s.pbtype != 0 && !(fabs(dn[0]) < 0.5 * s.L[0] && fabs(dn[1]) < 0.5 * s.L[1] && fabs(dn[2]) < 0.5 * s.L[2]))
+            mcx_mimg64(s, dn);
+        mcx_pair_force(s, dn, qa.x, qa.y, qa.z, pq.x, pq.y, pq.z, fnew);
+    }
 }
 
-__device__ __forceinline__ bool excluded(const DevSys &s, int i, int j)
+// Net nonbonded force on the moved set at its old position (force-bias
+// drift): pairs of a moved atom with an atom outside the set, over the same
+// list rows as move_delta; pairs inside the set cancel.  Result in m.fold
+// after the final barrier.
+__device__ void force_old(const DevSys &s, MoveLDS &m)
 {
-    const int lo = i < j ? i : j, hi = i < j ? j : i;
-    for (int k = s.excl_off[lo]; k < s.excl_off[lo + 1]; k++)
-        if (s.excl_idx[k] == hi) return true;
-    return false;
+    const int tid = threadIdx.x;
+    double f[3] = {0.0, 0.0, 0.0};
+    const int total = m.total, nm = m.nm;
+    for (int e = tid; e < total; e += kMCT) {
+        int lo = 0, hi = nm - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (m.pref[mid] <= e) lo = mid; else hi = mid - 1;
+        }
+        const int k = e - m.pref[lo];
+        const int i = m.first + lo;
+        const int j = s.cutmode == 0 ? k : s.nbr[(size_t)i * s.cap + k];
+        if (j >= m.first && j <= m.last) continue;
+        if (s.cutmode == 0 && excluded(s, i, j)) continue;
+        const int4 pi = m.ip[lo], pj = s.ipos[j];
+        double d[3];
+        if (s.pbtype != 0) {
+            d[0] = (double)mcx_wdiff(pj.x, pi.x) * s.unit[0];
+            d[1] = (double)mcx_wdiff(pj.y, pi.y) * s.unit[1];
+            d[2] = (double)mcx_wdiff(pj.z, pi.z) * s.unit[2];
+        } else {
+            d[0] = s.x[j] - s.x[i]; d[1] = s.y[j] - s.y[i]; d[2] = s.z[j] - s.z[i];
+        }
+        const double4 qa = m.mq[lo], pq = s.qes[j];
+        mcx_pair_force(s, d, qa.x, qa.y, qa.z, pq.x, pq.y, pq.z, f);
+    }
+    for (int c = 0; c < 3; c++) f[c] = wsum(f[c]);
+    const int w = tid >> 6;
+    if ((tid & 63) == 0) { m.red[3][w] = f[0]; m.red[4][w] = f[1]; m.red[5][w] = f[2]; }
+    __syncthreads();
+    if (tid < 3) {
+        double v = 0.0;
+        for (int k = 0; k < kMCW; k++) v += m.red[3 + tid][k];
+        m.fold[tid] = v;
+    }
+    __syncthreads();
 }
+
 
 // dE of the move described in m (trial coordinates, displacements and the
 // cached moved-atom data set by make_trial).  Wavefront 0 evaluates the
@@ -88,6 +144,8 @@ __device__ void move_delta(const DevSys &s, const DevTerms &t, const DevMoves &m
 {
     const int tid = threadIdx.x;
     double dec = 0.0, dev = 0.0, deb = 0.0;
+    double fnew[3] = {0.0, 0.0, 0.0};
+    const bool fb = m.type == MCX_MV_FBIAS_DEV;
     if (tid >= 64) {
         constexpr int NP = kMCT - 64;
         const int pt = tid - 64;
@@ -127,7 +185,7 @@ __device__ void move_delta(const DevSys &s, const DevTerms &t, const DevMoves &m
                     if (j < i) continue;     // pair of two moved atoms: counted once
                     b = j - m.first;
                 }
-                pair_delta(s, m, i, a, j, b, pj[u], pq[u], dec, dev);
+                pair_delta(s, m, i, a, j, b, pj[u], pq[u], dec, dev, fb, fnew);
             }
         }
     } else if (m.ra1 >= 0) {
@@ -178,11 +236,20 @@ __device__ void move_delta(const DevSys &s, const DevTerms &t, const DevMoves &m
     deb = wsum(deb);
     const int w = tid >> 6;
     if ((tid & 63) == 0) { m.red[0][w] = dec; m.red[1][w] = dev; m.red[2][w] = deb; }
+    if (fb) {
+        for (int c = 0; c < 3; c++) fnew[c] = wsum(fnew[c]);
+        if ((tid & 63) == 0) { m.red[3][w] = fnew[0]; m.red[4][w] = fnew[1]; m.red[5][w] = fnew[2]; }
+    }
     __syncthreads();
     if (tid == 0) {
         double c = 0.0, v = 0.0, bb = 0.0;
         for (int k = 0; k < kMCW; k++) { c += m.red[0][k]; v += m.red[1][k]; bb += m.red[2][k]; }
         m.de = MCX_CHGCHARMM * MCX_KCALTOIU * c + 4.0 * v + bb;
+    }
+    if (fb && tid >= 1 && tid <= 3) {
+        double v = 0.0;
+        for (int k = 0; k < kMCW; k++) v += m.red[2 + tid][k];
+        m.fnew[tid - 1] = v;
     }
     __syncthreads();
 }
@@ -217,15 +284,14 @@ __device__ void load_moved(const DevSys &s, const DevMoves &mv, MoveLDS &m)
 }
 
 // Phase C: trial coordinates (MOVE_TRN::tr_a_b / MOVE_ROT::apply_rotation,
-// include/Move.hpp:42-50,120-143, each atom re-wrapped), displacements and
-// the prefix of the moved atoms' list rows.
+// include/Move.hpp:42-50,120-143, each atom re-wrapped) and displacements.
 __device__ void make_trial(const DevSys &s, MoveLDS &m)
 {
     const int a = threadIdx.x;
     if (a < m.nm) {
         const double x = m.ox[a], y = m.oy[a], z = m.oz[a];
         double ux, uy, uz;
-        if (m.type == MCX_MV_TRN_DEV) {
+        if (m.type != MCX_MV_ROT_DEV) {   // TRN or force-bias translation
             ux = x + m.r[0]; uy = y + m.r[1]; uz = z + m.r[2];
         } else {
             double piv[3];
@@ -250,6 +316,13 @@ __device__ void make_trial(const DevSys &s, MoveLDS &m)
         m.dx[a] = ux - x; m.dy[a] = uy - y; m.dz[a] = uz - z;
         m.tx[a] = wrap1(s, ux, 0); m.ty[a] = wrap1(s, uy, 1); m.tz[a] = wrap1(s, uz, 2);
     }
+    __syncthreads();
+}
+
+// Prefix of the moved atoms' list-row lengths (the flattened pair index of
+// move_delta and force_old).
+__device__ void row_prefix(MoveLDS &m)
+{
     if (threadIdx.x == 64) {
         int acc = 0;
         for (int b = 0; b < m.nm; b++) { m.pref[b] = acc; acc += m.mc[b]; }
@@ -294,6 +367,21 @@ __global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s, DevTerms t, DevMove
         if (w0) pick_move(mv, dr, dmax, m, lane);
         __syncthreads();
         load_moved(s, mv, m);
+        row_prefix(m);
+        if (m.type == MCX_MV_FBIAS_DEV) {
+            // force-bias proposal: dx = beta A F + sqrt(2A) xi with F the net
+            // nonbonded force on the set in kcal/mol/A (iu/A / 418.4), A =
+            // dmax (A^2) and beta = 1/(R T) in mol/kcal
+            force_old(s, m);
+            if (tid == 0) {
+                const double A = dmax[m.acc], bA = arg.beta * A / MCX_KCALTOIU, sq = sqrt(2.0 * A);
+                for (int c = 0; c < 3; c++) {
+                    m.xi[c] = m.r[c];
+                    m.r[c] = bA * m.fold[c] + sq * m.xi[c];
+                }
+            }
+            __syncthreads();
+        }
         make_trial(s, m);
         move_delta(s, t, mv, m);
         // ---- phase E+F (wavefront 0): Metropolis test (apply_criterion,
@@ -302,7 +390,28 @@ __global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s, DevTerms t, DevMove
         if (w0) {
             const int tp = m.acc;
             const double de = m.de;
-            const int acc = metropolis(dr, arg.beta, de, lane);
+            int acc;
+            double lnq = 0.0;
+            if (m.type == MCX_MV_FBIAS_DEV) {
+                // min(1, exp(-beta dE) q(new->old)/q(old->new)), q the
+                // Gaussian of the proposal: log ratio =
+                // (|dx - bA F_old|^2 - |-dx - bA F_new|^2) / (4A); beta dE
+                // as the reference's test forms it (dE in iu), so force-bias
+                // and TRN/ROT moves leave the same density invariant
+                // whatever the drift's units
+                const double A = dmax[tp], bA = arg.beta * A / MCX_KCALTOIU;
+                double fw = 0.0, rv = 0.0;
+                for (int c = 0; c < 3; c++) {
+                    const double a = m.r[c] - bA * m.fold[c], b = -m.r[c] - bA * m.fnew[c];
+                    fw += a * a;
+                    rv += b * b;
+                }
+                lnq = (fw - rv) / (4.0 * A);
+                const double x = lnq - arg.beta * de;
+                acc = x >= 0.0 ? 1 : (dr.next(lane) < exp(x));
+            } else {
+                acc = metropolis(dr, arg.beta, de, lane);
+            }
             if (lane == 0) {
                 n_trial[tp]++;
                 if (acc) { etot += de; n_acc[tp]++; }
@@ -311,6 +420,11 @@ __global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s, DevTerms t, DevMove
                 if (arg.de_out) arg.de_out[o] = de;
                 if (arg.mvtyp_out) arg.mvtyp_out[o] = tp;
                 if (arg.mvinst_out) arg.mvinst_out[o] = m.inst;
+                if (arg.lnq_out) arg.lnq_out[o] = lnq;
+                if (arg.disp_out) {
+                    const bool tr = m.type != MCX_MV_ROT_DEV;
+                    for (int c = 0; c < 3; c++) arg.disp_out[3 * o + c] = tr ? m.r[c] : 0.0;
+                }
                 adapt_dmax(mv, st, tp, acc, dmax, acc_tmp);
             }
             if (acc && lane < m.nm) {

@@ -212,7 +212,9 @@ __device__ __forceinline__ double wrap1(const DevSys &s, double v, int k)
 // Phase A of a move (src/MC_metropolis.cpp:85-116; src/MC.cpp:137-156), run
 // by wavefront 0 with its lanes in step: move type, instance, the rndSphere
 // direction (rejection in the unit ball) and, for rotations, the angle and
-// the rotation matrix.  Lane 0 publishes the move in m.
+// the rotation matrix.  A force-bias move draws a Gaussian vector instead
+// (Box-Muller; left in m.r for the drift to be added).  Lane 0 publishes the
+// move in m.
 template <class M>
 __device__ void pick_move(const DevMoves &mv, WaveDraws &dr, const double *dmax, M &m, int lane)
 {
@@ -220,17 +222,26 @@ __device__ void pick_move(const DevMoves &mv, WaveDraws &dr, const double *dmax,
     const int in = (int)((double)mv.ninst[tp] * dr.next(lane));
     const int4 rec = mv.inst[mv.off[tp] + in];
     const int2 rr = mv.inst_adj[mv.off[tp] + in];
-    double r[3];
-    do {
-        r[0] = dr.next(lane) * 2.0 + (-1.0);
-        r[1] = dr.next(lane) * 2.0 + (-1.0);
-        r[2] = dr.next(lane) * 2.0 + (-1.0);
-    } while ((r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) > 1.0);
     const int type = mv.type[tp];
+    double r[3];
     double R[3][3];
+    if (type == MCX_MV_FBIAS_DEV) {
+        // force bias: three standard normals by Box-Muller from four draws
+        const double u1 = dr.next(lane), u2 = dr.next(lane), u3 = dr.next(lane), u4 = dr.next(lane);
+        const double a = sqrt(-2.0 * log(1.0 - u1)), b = sqrt(-2.0 * log(1.0 - u3));
+        r[0] = a * cos(MCX_TWOPI * u2);
+        r[1] = a * sin(MCX_TWOPI * u2);
+        r[2] = b * cos(MCX_TWOPI * u4);
+    } else {
+        do {
+            r[0] = dr.next(lane) * 2.0 + (-1.0);
+            r[1] = dr.next(lane) * 2.0 + (-1.0);
+            r[2] = dr.next(lane) * 2.0 + (-1.0);
+        } while ((r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) > 1.0);
+    }
     if (type == MCX_MV_TRN_DEV) {
         r[0] = r[0] * dmax[tp]; r[1] = r[1] * dmax[tp]; r[2] = r[2] * dmax[tp];
-    } else {
+    } else if (type == MCX_MV_ROT_DEV) {
         const double rang = dmax[tp] * (dr.next(lane) * 2.0 + (-1.0));
         rot_mat(R, r, rang);
     }
@@ -240,7 +251,7 @@ __device__ void pick_move(const DevMoves &mv, WaveDraws &dr, const double *dmax,
         m.ra0 = rr.x; m.ra1 = rr.y;
         m.nm = rec.y - rec.x + 1;
         m.r[0] = r[0]; m.r[1] = r[1]; m.r[2] = r[2];
-        if (type != MCX_MV_TRN_DEV)
+        if (type == MCX_MV_ROT_DEV)
             for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) m.R[a][b] = R[a][b];
     }
 }
@@ -268,6 +279,37 @@ __device__ __forceinline__ void adapt_dmax(const DevMoves &mv, long long st, int
             acc_tmp[k] = 0;
         }
     }
+}
+
+// Force on atom i from atom j, FP64: d = x_j - x_i (minimum image), the
+// pair energy of mcx_pair_change; adds -dE_pair/dx_i = 2 (dE/dr^2) d to f
+// (iu/A).  Nothing is added beyond the cutoff.
+__device__ __forceinline__ void mcx_pair_force(const DevSys &s, const double d[3], double qi, double ei, double si,
+                                               double qj, double ej, double sj, double f[3])
+{
+    const double r2 = fma(d[2], d[2], fma(d[1], d[1], d[0] * d[0]));
+    const bool full = s.cutmode == 0;
+    if (!full && !(r2 <= s.ctoff2)) return;
+    float f32;
+    const double ri = mcx_rsqrt(r2, f32);
+    const double ri2 = ri * ri;
+    const double sij = si + sj;
+    const double t = sij * sij * ri2;
+    const double t3 = t * t * t;
+    const double C = MCX_CHGCHARMM * MCX_KCALTOIU;
+    const double qq = C * qi * qj, ee = 4.0 * ei * ej;
+    const double E0 = qq * ri + ee * fma(t3, t3, -t3);
+    const double dE0 = -0.5 * qq * ri * ri2 + ee * fma(-6.0, t3 * t3, 3.0 * t3) * ri2;   // dE0/dr^2
+    double S = 1.0, dS = 0.0;
+    if (!full && r2 > s.cton2) {
+        const double a = s.ctoff2 - r2, b = s.ctoff2 + 2.0 * r2 - 3.0 * s.cton2;
+        S = a * a * b * s.switch2;
+        dS = (-2.0 * a * b + 2.0 * a * a) * s.switch2;
+    }
+    const double g = S * dE0 + dS * E0;
+    f[0] += 2.0 * g * d[0];
+    f[1] += 2.0 * g * d[1];
+    f[2] += 2.0 * g * d[2];
 }
 
 // Change of one pair's nonbonded sums when atom i and possibly j move.

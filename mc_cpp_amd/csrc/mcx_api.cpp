@@ -430,8 +430,10 @@ DevMoves build_moves(mcx_ctx *h, const mcx_move_type *types, int ntypes, MoveBuf
     const std::vector<int> resLast = residue_last_table(h);
     for (int t = 0; t < ntypes; t++) {
         const mcx_move_type &m = types[t];
-        if (m.type != MCX_MV_TRN && m.type != MCX_MV_ROT) throw mcx_error(MCX_ERR_ARG, "move type must be TRN or ROT");
+        if (m.type != MCX_MV_TRN && m.type != MCX_MV_ROT && m.type != MCX_MV_FBIAS)
+            throw mcx_error(MCX_ERR_ARG, "move type must be TRN, ROT or FBIAS");
         if (m.mode == MCX_MODE_ATOM && m.type == MCX_MV_ROT) throw mcx_error(MCX_ERR_ARG, "rotation of a single atom");
+        if (m.type == MCX_MV_FBIAS && !(m.dmax > 0.0)) throw mcx_error(MCX_ERR_ARG, "force-bias step A must be > 0");
         mv.type[t] = m.type;
         mv.dmax0[t] = m.dmax;
         mv.target[t] = m.target;
@@ -849,6 +851,18 @@ int mcx_mc_run(mcx_ctx *h, const mcx_move_type *types, int ntypes, int nsteps, u
                int list_every, mcx_mc_stats *stats, unsigned char *acc_out, double *de_out, int *mvtyp_out,
                int *mvinst_out)
 {
+    mcx_mc_trace tr{acc_out, de_out, mvtyp_out, mvinst_out, nullptr, nullptr};
+    return mcx_mc_run_traced(h, types, ntypes, nsteps, seed, rng_mode, list_every, stats, &tr);
+}
+
+int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int nsteps, uint64_t seed, int rng_mode,
+                      int list_every, mcx_mc_stats *stats, const mcx_mc_trace *trace)
+{
+    const mcx_mc_trace none{};
+    const mcx_mc_trace &tr = trace ? *trace : none;
+    unsigned char *acc_out = tr.acc;
+    double *de_out = tr.de, *disp_out = tr.disp, *lnq_out = tr.lnq;
+    int *mvtyp_out = tr.mvtyp, *mvinst_out = tr.mvinst;
     return guarded(h, [&] {
         if (nsteps < 0) throw mcx_error(MCX_ERR_ARG, "nsteps < 0");
         if (!h->have_excl || !h->have_coords || !h->have_atoms) throw mcx_error(MCX_ERR_STATE, "system not set up");
@@ -862,12 +876,14 @@ int mcx_mc_run(mcx_ctx *h, const mcx_move_type *types, int ntypes, int nsteps, u
         DBuf<unsigned char> acc_d;
         DBuf<double> de_d;
         DBuf<int> typ_d, inst_d;
+        DBuf<double> disp_d, lnq_d;
         struct Cleanup {
             std::vector<std::function<void()>> f;
             ~Cleanup() { for (auto &g : f) g(); }
         } cleanup;
         cleanup.f.push_back([&] { etot.release(); dmax.release(); mts.release(); trials.release(); accepted.release();
-                                  acc_tmp.release(); acc_d.release(); de_d.release(); typ_d.release(); inst_d.release(); });
+                                  acc_tmp.release(); acc_d.release(); de_d.release(); typ_d.release(); inst_d.release();
+                                  disp_d.release(); lnq_d.release(); });
         double ener[10], r9[9];
         energy_device(h, r9);
         assemble(r9, ener);
@@ -889,6 +905,8 @@ int mcx_mc_run(mcx_ctx *h, const mcx_move_type *types, int ntypes, int nsteps, u
         if (de_out) de_d.alloc(std::max(nsteps, 1));
         if (mvtyp_out) typ_d.alloc(std::max(nsteps, 1));
         if (mvinst_out) inst_d.alloc(std::max(nsteps, 1));
+        if (disp_out) disp_d.alloc((size_t)3 * std::max(nsteps, 1));
+        if (lnq_out) lnq_d.alloc(std::max(nsteps, 1));
         MCRunArgs a{};
         a.rng_mode = rng_mode == MCX_MC_REPLAY ? 1 : 0;
         a.seed = seed;
@@ -900,6 +918,8 @@ int mcx_mc_run(mcx_ctx *h, const mcx_move_type *types, int ntypes, int nsteps, u
         a.de_out = de_out ? de_d.p : nullptr;
         a.mvtyp_out = mvtyp_out ? typ_d.p : nullptr;
         a.mvinst_out = mvinst_out ? inst_d.p : nullptr;
+        a.disp_out = disp_out ? disp_d.p : nullptr;
+        a.lnq_out = lnq_out ? lnq_d.p : nullptr;
         HIPCHK(hipStreamSynchronize(h->stream));
         const auto t0 = std::chrono::steady_clock::now();
         long long nlist = 0;
@@ -934,6 +954,9 @@ int mcx_mc_run(mcx_ctx *h, const mcx_move_type *types, int ntypes, int nsteps, u
         if (de_out && nsteps) HIPCHK(hipMemcpy(de_out, de_d.p, nsteps * sizeof(double), hipMemcpyDeviceToHost));
         if (mvtyp_out && nsteps) HIPCHK(hipMemcpy(mvtyp_out, typ_d.p, nsteps * sizeof(int), hipMemcpyDeviceToHost));
         if (mvinst_out && nsteps) HIPCHK(hipMemcpy(mvinst_out, inst_d.p, nsteps * sizeof(int), hipMemcpyDeviceToHost));
+        if (disp_out && nsteps)
+            HIPCHK(hipMemcpy(disp_out, disp_d.p, (size_t)3 * nsteps * sizeof(double), hipMemcpyDeviceToHost));
+        if (lnq_out && nsteps) HIPCHK(hipMemcpy(lnq_out, lnq_d.p, nsteps * sizeof(double), hipMemcpyDeviceToHost));
     });
 }
 
@@ -962,6 +985,84 @@ int mcx_nonbonded_grad(mcx_ctx *h, double *gx, double *gy, double *gz, double *g
     });
 }
 
+int mcx_nonbonded_grad_types(mcx_ctx *h, int ntypes, const int *type, double *gx, double *gy, double *gz,
+                             double *geps, double *gsig, double *gq)
+{
+    return guarded(h, [&] {
+        if (h->cutmode != MCX_CUT_SWITCH) throw mcx_error(MCX_ERR_UNSUPPORTED, "gradients need SWITCH mode");
+        if (ntypes < 1 || !type) throw mcx_error(MCX_ERR_ARG, "atom types required");
+        const int n = h->n;
+        for (int i = 0; i < n; i++)
+            if (type[i] < 0 || type[i] >= ntypes) throw mcx_error(MCX_ERR_ARG, "atom type out of range");
+        if (!h->list_valid) build_list(h);
+        DBuf<double> g, out;
+        DBuf<int> ty;
+        struct Release {
+            DBuf<double> &a, &b;
+            DBuf<int> &c;
+            ~Release() { a.release(); b.release(); c.release(); }
+        } rel{g, out, ty};
+        g.alloc((size_t)6 * n);
+        out.alloc((size_t)3 * ntypes);
+        ty.upload(type, n, h->stream);
+        launch_nonbonded_grad(h->sys(), g.p, h->stream);
+        launch_type_sums(g.p, n, ty.p, ntypes, out.p, h->stream);
+        HIPCHK(hipGetLastError());
+        std::vector<double> hg((size_t)3 * n), ho((size_t)3 * ntypes);
+        HIPCHK(hipMemcpyAsync(hg.data(), g.p, hg.size() * 8, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(ho.data(), out.p, ho.size() * 8, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        double *pos[3] = {gx, gy, gz};
+        for (int c = 0; c < 3; c++)
+            if (pos[c]) std::copy(hg.begin() + (size_t)c * n, hg.begin() + (size_t)(c + 1) * n, pos[c]);
+        for (int t = 0; t < ntypes; t++) {
+            if (geps) geps[t] = ho[3 * t];
+            if (gsig) gsig[t] = ho[3 * t + 1];
+            if (gq) gq[t] = ho[3 * t + 2];
+        }
+    });
+}
+
+int mcx_nonbonded_grad_batch(mcx_ctx *const *hs, int nsys, double *const *g6)
+{
+    if (!hs || nsys < 1 || !g6) return MCX_ERR_ARG;
+    for (int k = 0; k < nsys; k++)
+        if (!hs[k] || !g6[k]) return MCX_ERR_ARG;
+    mcx_ctx *h0 = hs[0];
+    return guarded(h0, [&] {
+        std::vector<DevSys> d(nsys);
+        std::vector<DBuf<double>> gb(nsys);
+        DBuf<DevSys> dd;
+        DBuf<double *> gp;
+        struct Release {
+            std::vector<DBuf<double>> &a;
+            DBuf<DevSys> &b;
+            DBuf<double *> &c;
+            ~Release() { for (auto &x : a) x.release(); b.release(); c.release(); }
+        } rel{gb, dd, gp};
+        int nmax = 0;
+        std::vector<double *> ptrs(nsys);
+        for (int k = 0; k < nsys; k++) {
+            mcx_ctx *h = hs[k];
+            if (h->device != h0->device) throw mcx_error(MCX_ERR_ARG, "systems on different devices");
+            if (h->cutmode != MCX_CUT_SWITCH) throw mcx_error(MCX_ERR_UNSUPPORTED, "gradients need SWITCH mode");
+            if (!h->list_valid) build_list(h);
+            HIPCHK(hipStreamSynchronize(h->stream));
+            d[k] = h->sys();
+            gb[k].alloc((size_t)6 * h->n);
+            ptrs[k] = gb[k].p;
+            nmax = std::max(nmax, h->n);
+        }
+        dd.upload(d.data(), nsys, h0->stream);
+        gp.upload(ptrs.data(), nsys, h0->stream);
+        launch_nonbonded_grad_batch(dd.p, gp.p, nsys, nmax, h0->stream);
+        HIPCHK(hipGetLastError());
+        for (int k = 0; k < nsys; k++)
+            HIPCHK(hipMemcpyAsync(g6[k], gb[k].p, (size_t)6 * hs[k]->n * 8, hipMemcpyDeviceToHost, h0->stream));
+        HIPCHK(hipStreamSynchronize(h0->stream));
+    });
+}
+
 int mcx_device_coords(mcx_ctx *h, int which, void **ptr)
 {
     return guarded(h, [&] {
@@ -976,16 +1077,24 @@ int mcx_time_kernel(mcx_ctx *h, int which, int reps, double *ms)
         if (reps < 1) throw mcx_error(MCX_ERR_ARG, "reps < 1");
         DevSys s = h->sys();
         DevTerms t = h->dterms();
-        if (which == 1 || which == 2) { if (!h->list_valid) build_list(h); s = h->sys(); }
+        if (which == 1 || which == 2 || which == 4) { if (!h->list_valid) build_list(h); s = h->sys(); }
+        if (which == 4 && h->cutmode != MCX_CUT_SWITCH) throw mcx_error(MCX_ERR_UNSUPPORTED, "gradients need SWITCH mode");
         if (which == 0 && h->ntiles == 0) setup_tiles(h);
         const int nbs = switch_energy_blocks(h->n);
         h->partial.alloc((size_t)std::max({h->ntiles * 2, nbs * 2, bonded_blocks(t) * 7, 16}));
+        DBuf<double> g6;
+        struct Release {
+            DBuf<double> &a;
+            ~Release() { a.release(); }
+        } rel{g6};
+        if (which == 4) g6.alloc((size_t)6 * h->n);
         auto launch = [&] {
             switch (which) {
             case 0: launch_full_energy(s, h->tiles.p, h->ntiles, 256, h->partial.p, h->stream); break;
             case 1: launch_switch_energy(s, h->partial.p, nbs, h->stream); break;
             case 2: launch_list_build(s, h->grid, h->stream); break;
             case 3: launch_bonded(s, t, h->partial.p, bonded_blocks(t), h->stream); break;
+            case 4: launch_nonbonded_grad(s, g6.p, h->stream); break;
             default: throw mcx_error(MCX_ERR_ARG, "unknown kernel id");
             }
         };

@@ -402,6 +402,9 @@ int mcx_batch_create(mcx_ctx *const *reps, int nrep, const mcx_move_type *types,
         for (int r = 0; r < nrep; r++) {
             b->types[r].assign(types + at, types + at + ntypes[r]);
             at += ntypes[r];
+            for (const auto &t : b->types[r])
+                if (t.type == MCX_MV_FBIAS)
+                    throw mcx_error(MCX_ERR_UNSUPPORTED, "force-bias moves run in mcx_mc_run only");
         }
         b->mbufs.resize(nrep);
         b->moves.resize(nrep);

@@ -26,6 +26,7 @@
 
 #define MCX_MV_TRN_DEV 1   // move type codes as in include/Move.hpp:33-38
 #define MCX_MV_ROT_DEV 2
+#define MCX_MV_FBIAS_DEV 3 // force-bias translation (not in the reference)
 
 constexpr int kMaxMoveAtoms = 64;   // atoms per move instance on the device path
 constexpr int kMaxMoveTypes = 8;
@@ -166,11 +167,15 @@ struct MCRunArgs {
     long long *trials, *accepted;   // [ntypes] device
     int *acc_tmp;          // [ntypes]
     unsigned char *acc_out; double *de_out; int *mvtyp_out; int *mvinst_out;  // device or null
+    double *disp_out;      // [3*nsteps] translation of TRN and force-bias moves, or null
+    double *lnq_out;       // [nsteps] log proposal-density ratio of force-bias moves, or null
 };
 void launch_mc_chunk(const DevSys &s, const DevTerms &t, const DevMoves &mv, const MCRunArgs &a,
                      hipStream_t st);
 // adjoint.hip
 void launch_nonbonded_grad(const DevSys &s, double *g6, hipStream_t st);
+void launch_nonbonded_grad_batch(const DevSys *sys, double *const *g6, int nsys, int nmax, hipStream_t st);
+void launch_type_sums(const double *g6, int n, const int *type, int ntypes, double *out, hipStream_t st);
 // batch.hip: one Metropolis chain per replica, one workgroup each
 constexpr int kBatchTypes = 64;     // distinct (q, eps, sig) atom types of a replica
 constexpr int kBatchAtoms = 32;     // atoms per move instance

@@ -57,6 +57,11 @@ class BatchStatsC(C.Structure):
 
 _llp = C.POINTER(C.c_longlong)
 
+
+class MCTraceC(C.Structure):
+    _fields_ = [("acc", C.POINTER(C.c_ubyte)), ("de", _dp), ("mvtyp", _ip), ("mvinst", _ip), ("disp", _dp),
+                ("lnq", _dp)]
+
 # every symbol include/mcx.h declares, with its argument types
 SIGNATURES = {
     "mcx_create": ([C.c_int, C.c_int, C.POINTER(C.c_void_p)], C.c_int),
@@ -81,7 +86,11 @@ SIGNATURES = {
     "mcx_delta_energy": ([C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp], C.c_int),
     "mcx_mc_run": ([C.c_void_p, C.POINTER(MoveTypeC), C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int,
                     C.POINTER(MCStatsC), C.POINTER(C.c_ubyte), _dp, _ip, _ip], C.c_int),
+    "mcx_mc_run_traced": ([C.c_void_p, C.POINTER(MoveTypeC), C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int,
+                           C.POINTER(MCStatsC), C.POINTER(MCTraceC)], C.c_int),
     "mcx_nonbonded_grad": ([C.c_void_p] + [_dp] * 6, C.c_int),
+    "mcx_nonbonded_grad_types": ([C.c_void_p, C.c_int, _ip] + [_dp] * 6, C.c_int),
+    "mcx_nonbonded_grad_batch": ([C.POINTER(C.c_void_p), C.c_int, C.POINTER(_dp)], C.c_int),
     "mcx_device_coords": ([C.c_void_p, C.c_int, C.POINTER(C.c_void_p)], C.c_int),
     "mcx_time_kernel": ([C.c_void_p, C.c_int, C.c_int, _dp], C.c_int),
     "mcx_set_rng_stream": ([C.c_void_p, C.c_uint32], C.c_int),
@@ -247,6 +256,16 @@ class Handle:
         self._chk(self.L.mcx_nonbonded_grad(self.h, *[_p(a, C.c_double) for a in out]))
         return out
 
+    def grad_types(self, type_of_atom, ntypes=None):
+        """Forces per atom and parameter gradients summed per type."""
+        ty = _i(type_of_atom)
+        nt = int(ty.max()) + 1 if ntypes is None else int(ntypes)
+        gx, gy, gz = np.zeros(self.n), np.zeros(self.n), np.zeros(self.n)
+        ge, gs, gq = np.zeros(nt), np.zeros(nt), np.zeros(nt)
+        self._chk(self.L.mcx_nonbonded_grad_types(self.h, nt, _p(ty, C.c_int), *[_p(a, C.c_double) for a in
+                                                                                (gx, gy, gz, ge, gs, gq)]))
+        return gx, gy, gz, ge, gs, gq
+
     def time_kernel(self, which, reps=20):
         ms = C.c_double(0)
         self._chk(self.L.mcx_time_kernel(self.h, int(which), int(reps), C.byref(ms)))
@@ -263,13 +282,17 @@ class Handle:
         de = np.zeros(max(nsteps, 1)) if record else None
         ty = np.zeros(max(nsteps, 1), np.int32) if record else None
         ins = np.zeros(max(nsteps, 1), np.int32) if record else None
-        self._chk(self.L.mcx_mc_run(self.h, arr, nt, int(nsteps), C.c_uint64(seed), int(rng_mode), int(list_every),
-                                    C.byref(st), _p(acc, C.c_ubyte), _p(de, C.c_double), _p(ty, C.c_int),
-                                    _p(ins, C.c_int)))
+        disp = np.zeros((max(nsteps, 1), 3)) if record else None
+        lnq = np.zeros(max(nsteps, 1)) if record else None
+        tr = MCTraceC(_p(acc, C.c_ubyte), _p(de, C.c_double), _p(ty, C.c_int), _p(ins, C.c_int),
+                      _p(disp, C.c_double), _p(lnq, C.c_double))
+        self._chk(self.L.mcx_mc_run_traced(self.h, arr, nt, int(nsteps), C.c_uint64(seed), int(rng_mode),
+                                           int(list_every), C.byref(st), C.byref(tr)))
         out = dict(etot=st.etot, trials=np.array(st.trials[:nt]), accepted=np.array(st.accepted[:nt]),
                    dmax=np.array(st.dmax[:nt]), list_updates=st.list_updates, seconds=st.seconds)
         if record:
-            out.update(acc=acc[:nsteps], de=de[:nsteps], mvtyp=ty[:nsteps], mvinst=ins[:nsteps])
+            out.update(acc=acc[:nsteps], de=de[:nsteps], mvtyp=ty[:nsteps], mvinst=ins[:nsteps],
+                       disp=disp[:nsteps], lnq=lnq[:nsteps])
         return out
 
 
@@ -281,6 +304,19 @@ def _move_array(moves: Sequence[S.MoveType]):
         if m.sele is not None:
             sa = _i(m.sele); keep.append(sa); arr[k].sele = _p(sa, C.c_int)
     return arr, keep
+
+
+def grad_batch(handles: Sequence[Handle]):
+    """Gradients of many systems in one launch: a list of (gx, gy, gz, geps,
+    gsig, gq) per system."""
+    L = load_library()
+    hs = (C.c_void_p * len(handles))(*[h.h for h in handles])
+    outs = [np.zeros(6 * h.n) for h in handles]
+    ptrs = (_dp * len(handles))(*[_p(o, C.c_double) for o in outs])
+    rc = L.mcx_nonbonded_grad_batch(hs, len(handles), ptrs)
+    if rc != MCX_OK:
+        raise MCXError(rc, L.mcx_last_error(handles[0].h).decode())
+    return [tuple(o.reshape(6, h.n)) for o, h in zip(outs, handles)]
 
 
 def from_system(s: S.System, device: int = 0) -> Handle:

@@ -25,7 +25,7 @@ RBOLTZUI = 0.83144621     # include/Constants.hpp:35
 
 PBC_NONE, PBC_CUBIC, PBC_ORBIC = 0, 1, 2
 CUT_FULL, CUT_SWITCH = 0, 1
-MV_TRN, MV_ROT = 1, 2
+MV_TRN, MV_ROT, MV_FBIAS = 1, 2, 3   # MV_FBIAS: force-bias translation (not in the reference)
 MODE_RESIDUE, MODE_ALL, MODE_ATOM = 1, 2, 4
 
 
@@ -111,8 +111,11 @@ class System:
         self.resid = _i(self.resid)
 
     def copy(self) -> "System":
+        """Deep copy of every per-atom array and term table (moves are shared)."""
+        arrays = {f.name: getattr(self, f.name).copy() for f in dataclasses.fields(self)
+                  if isinstance(getattr(self, f.name), np.ndarray)}
         return dataclasses.replace(
-            self, x=self.x.copy(), y=self.y.copy(), z=self.z.copy(),
+            self, **arrays,
             bonds={k: v.copy() for k, v in self.bonds.items()},
             ub={k: v.copy() for k, v in self.ub.items()},
             angles={k: v.copy() for k, v in self.angles.items()},
