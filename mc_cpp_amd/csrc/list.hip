@@ -9,22 +9,47 @@
 // summed left to right, no contraction.  The result is the reference's set
 // bit for bit.
 //
-// Layout: atoms are binned into a periodic cell grid (FP64, cells at least
-// (ctoff+dcut)/reach wide); then one wavefront per atom i tests the atoms of
-// the surrounding cells and sets bit j of an LDS bitmap for every hit.
-// Scanning the bitmap in word order is the stream compaction: it writes the
-// row's j in ascending order into an ELL row of capacity `cap` (full list,
-// both directions; the j > i suffix is the reference's half row).
+// Binning: atoms go into a periodic cell grid (cells at least
+// (ctoff+dcut)/reach wide) by one workgroup with LDS counters when the grid
+// has at most kBinCells cells (one launch), else by global counting kernels;
+// then each cell's atoms are put in ascending order, which makes the list
+// rows (and every sum over them) the same from build to build.
+//
+// Rows: a workgroup of four wavefronts takes a group of up to G atoms of
+// one cell and streams the surrounding cells' atoms once for all G rows
+// (cell-ordered records with 2*atom + frozen).  In a periodic box each
+// candidate is first tested with an FP32 r^2 from the fixed-point mirror,
+// taken about the cell centre when the box is large enough for that to be
+// every row atom's minimum image; a candidate with any FP32 r^2 within
+// 1e-5 (relative) of (ctoff+dcut)^2 -- the FP32 chain's own error is below
+// 1e-6 -- is decided by the FP64 test on the FP64 coordinates.  Without
+// periodic boundaries every candidate takes the FP64 test.  Every hit sets
+// bit j of the row's LDS bitmap; scanning the bitmap in word order is the
+// stream compaction: it writes the row's j in ascending order into an ELL
+// row of capacity `cap` (full list, both directions; the j > i suffix from
+// half[i] on is the reference's half row).  Each wavefront compacts every
+// fourth row of the group.
 #include "mcx_internal.h"
+
+#include <cmath>
 
 #pragma clang fp contract(off)
 
 namespace {
 
+constexpr int kBinCells = 8192;    // LDS counters of the one-launch binning
+constexpr int kMaxRuns = 64;       // 2 runs for each of up to 25 (y, z) cell columns, rounded up
+constexpr int kUnroll = 4;
+constexpr int kRowWaves = 4;       // wavefronts per row-group workgroup
+constexpr double kBand = 1e-5;     // relative band around (ctoff+dcut)^2 decided in FP64
+
 __device__ __forceinline__ double ref_rint(double v)
 {
     return (double)(v >= 0. ? (int)(v + 0.5) : (int)(v - 0.5));
 }
+
+// wrapping int32 difference b - a: the minimum image in units of L/2^32
+__device__ __forceinline__ int wdiff(int b, int a) { return (int)((unsigned)b - (unsigned)a); }
 
 __global__ void k_zero_int(int *p, int n)
 {
@@ -39,44 +64,115 @@ __device__ __forceinline__ int cell_index(double x, double rL, int nc)
     return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
 }
 
-__global__ void k_cell_count(DevSys s, CellGrid g)
+__device__ __forceinline__ int cell_of_atom(const DevSys &s, const CellGrid &g, int a)
 {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= s.n) return;
-    int c = 0;
-    if (g.ncell > 1) {
-        const int cx = cell_index(s.x[a], s.rL[0], g.nc[0]);
-        const int cy = cell_index(s.y[a], s.rL[1], g.nc[1]);
-        const int cz = cell_index(s.z[a], s.rL[2], g.nc[2]);
-        c = (cz * g.nc[1] + cy) * g.nc[0] + cx;
-    }
-    g.cell_of[a] = c;
-    atomicAdd(&g.cell_count[c], 1);
+    if (g.ncell == 1) return 0;
+    const int cx = cell_index(s.x[a], s.rL[0], g.nc[0]);
+    const int cy = cell_index(s.y[a], s.rL[1], g.nc[1]);
+    const int cz = cell_index(s.z[a], s.rL[2], g.nc[2]);
+    return (cz * g.nc[1] + cy) * g.nc[0] + cx;
 }
 
-// exclusive scan of cell counts, one block
-__global__ __launch_bounds__(1024) void k_cell_scan(CellGrid g)
+// the cell-ordered record of atom a at position p
+__device__ __forceinline__ void store_record(const DevSys &s, const CellGrid &g, int a, int p)
 {
-    __shared__ int sm[1024];
-    __shared__ int carry;
-    if (threadIdx.x == 0) carry = 0;
+    g.cell_atoms[p] = a;
+    const int code = 2 * a + (s.frozen[a] ? 1 : 0);
+    if (s.pbtype != 0) {
+        const int4 q = s.ipos[a];
+        g.cfix[p] = make_int4(q.x, q.y, q.z, code);
+    } else {
+        g.cpos[p] = make_double4(s.x[a], s.y[a], s.z[a], (double)code);
+    }
+}
+
+// Block-wide exclusive scan of f(in[c]) for c in [0, n) into out (and into
+// out2 when given; out2 may alias `in`: chunk c is read before it is
+// written), 1024 threads; returns the total.  sm: 1024 ints of LDS.
+template <class F>
+__device__ int block_scan(const int *in, int *out, int n, int *sm, int *carry, F f, int *out2 = nullptr)
+{
+    if (threadIdx.x == 0) *carry = 0;
     __syncthreads();
-    for (int base = 0; base < g.ncell; base += 1024) {
+    for (int base = 0; base < n; base += 1024) {
         const int c = base + threadIdx.x;
-        const int v = c < g.ncell ? g.cell_count[c] : 0;
+        const int v = c < n ? f(in[c]) : 0;
         sm[threadIdx.x] = v;
         __syncthreads();
         for (int o = 1; o < 1024; o <<= 1) {
-            int t = threadIdx.x >= o ? sm[threadIdx.x - o] : 0;
+            const int t = threadIdx.x >= o ? sm[threadIdx.x - o] : 0;
             __syncthreads();
             sm[threadIdx.x] += t;
             __syncthreads();
         }
-        if (c < g.ncell) g.cell_start[c] = carry + sm[threadIdx.x] - v;
+        if (c < n) {
+            out[c] = *carry + sm[threadIdx.x] - v;
+            if (out2) out2[c] = *carry + sm[threadIdx.x] - v;
+        }
         __syncthreads();
-        if (threadIdx.x == 1023) carry += sm[1023];
+        if (threadIdx.x == 1023) *carry += sm[1023];
         __syncthreads();
     }
+    const int total = *carry;
+    __syncthreads();      // every thread has read the total before the next reset
+    return total;
+}
+
+struct Identity {
+    __device__ int operator()(int v) const { return v; }
+};
+struct Groups {
+    int g;
+    __device__ int operator()(int v) const { return (v + g - 1) / g; }
+};
+
+// One-launch binning (ncell <= kBinCells): LDS counts, cell starts, row
+// groups (ceil(count / group) per cell), cell-ordered records; also clears
+// the overflow flag.
+__global__ __launch_bounds__(1024) void k_bin(DevSys s, CellGrid g, int group)
+{
+    __shared__ int cnt[kBinCells];
+    __shared__ int sm[1024];
+    __shared__ int carry;
+    const int nc = g.ncell;
+    for (int c = threadIdx.x; c < nc; c += 1024) cnt[c] = 0;
+    if (threadIdx.x == 0) s.overflow[0] = 0;
+    __syncthreads();
+    for (int a = threadIdx.x; a < s.n; a += 1024) {
+        const int c = cell_of_atom(s, g, a);
+        g.cell_of[a] = c;
+        atomicAdd(&cnt[c], 1);
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < nc; c += 1024) g.cell_count[c] = cnt[c];
+    const int ng = block_scan(cnt, g.group_start, nc, sm, &carry, Groups{group});
+    if (threadIdx.x == 0) g.group_start[nc] = ng;
+    // cell starts, also left in the LDS counters as the fill cursors
+    block_scan(cnt, g.cell_start, nc, sm, &carry, Identity{}, cnt);
+    // one cell: atom order; else unordered within cells (k_cell_sort)
+    for (int a = threadIdx.x; a < s.n; a += 1024) {
+        if (nc == 1) store_record(s, g, a, a);
+        else g.cell_tmp[atomicAdd(&cnt[g.cell_of[a]], 1)] = a;
+    }
+}
+
+// Binning for larger grids: global counters over several launches.
+__global__ void k_cell_count(DevSys s, CellGrid g)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= s.n) return;
+    const int c = cell_of_atom(s, g, a);
+    g.cell_of[a] = c;
+    atomicAdd(&g.cell_count[c], 1);
+}
+
+__global__ __launch_bounds__(1024) void k_cell_scan(CellGrid g, int group)
+{
+    __shared__ int sm[1024];
+    __shared__ int carry;
+    block_scan(g.cell_count, g.cell_start, g.ncell, sm, &carry, Identity{});
+    const int ng = block_scan(g.cell_count, g.group_start, g.ncell, sm, &carry, Groups{group});
+    if (threadIdx.x == 0) g.group_start[g.ncell] = ng;
 }
 
 __global__ void k_cell_fill(DevSys s, CellGrid g)
@@ -84,9 +180,24 @@ __global__ void k_cell_fill(DevSys s, CellGrid g)
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= s.n) return;
     const int c = g.cell_of[a];
-    const int p = g.cell_start[c] + atomicAdd(&g.cursor[c], 1);
-    g.cell_atoms[p] = a;
-    g.cpos[p] = make_double4(s.x[a], s.y[a], s.z[a], (double)(2 * a + (s.frozen[a] ? 1 : 0)));
+    g.cell_tmp[g.cell_start[c] + atomicAdd(&g.cursor[c], 1)] = a;
+}
+
+// Ascending atom order within each cell (one block per cell, rank sort:
+// atom indices are distinct), so that the cell records, the row groups
+// and the order of every list row do not depend on the order in which the
+// binning atomics landed.
+__global__ __launch_bounds__(256) void k_cell_sort(DevSys s, CellGrid g)
+{
+    const int c = blockIdx.x;
+    const int st = g.cell_start[c], cnt = g.cell_count[c];
+    const int *src = g.cell_tmp + st;
+    for (int e = threadIdx.x; e < cnt; e += 256) {
+        const int a = src[e];
+        int rank = 0;
+        for (int k = 0; k < cnt; k++) rank += src[k] < a;
+        store_record(s, g, a, st + rank);
+    }
 }
 
 // Tools::distance2 + the r2 <= cutnb2 test, operation by operation
@@ -105,95 +216,61 @@ __device__ __forceinline__ bool within(const DevSys &s, double4 pj, double xi, d
     return r2 <= s.cutnb2;
 }
 
-constexpr int kMaxRuns = 64;   // 2 runs for each of up to 25 (y, z) cell columns, rounded up
-constexpr int kUnroll = 4;
-
-template <bool PBC>
-__global__ __launch_bounds__(256) void k_build_rows(DevSys s, CellGrid g, int words, int wpb)
+// Candidate runs of the cells around cell ci: thread t < 32 takes (y, z)
+// column t of neighbour cells; the x cells of a column are contiguous in
+// cell order, except where the range wraps (then two runs).  Returns the
+// number of columns (2 runs each); the runs are visible after the caller's
+// barrier.
+__device__ __forceinline__ int cell_runs(const CellGrid &g, int ci, int t, int *run_start, int *run_len)
 {
-    extern __shared__ uint32_t bitmap[];
-    __shared__ int run_start[4][kMaxRuns], run_len[4][kMaxRuns];
-    const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    const int i = blockIdx.x * wpb + w;
-    const bool active = i < s.n;
-    uint32_t *B = bitmap + (size_t)w * words;
-    for (int k = lane; k < words; k += 64) B[k] = 0u;
-
-    // phase 1: the candidate runs of row i.  One lane per (y, z) column of
-    // neighbour cells; the x cells of a column are contiguous in cell
-    // order, except where the range wraps (then two runs).
-    double xi = 0.0, yi = 0.0, zi = 0.0;
-    bool fi = false;
-    int ncol = 0;
-    if (active) {
-        xi = s.x[i]; yi = s.y[i]; zi = s.z[i];
-        fi = s.frozen[i] != 0;
-        const int ci = g.cell_of[i];
-        const int cc[3] = {ci % g.nc[0], (ci / g.nc[0]) % g.nc[1], ci / (g.nc[0] * g.nc[1])};
-        int lo[3], cnt[3];
-        for (int d = 0; d < 3; d++) {
-            if (2 * g.reach[d] + 1 >= g.nc[d]) { lo[d] = 0; cnt[d] = g.nc[d]; }
-            else { lo[d] = cc[d] - g.reach[d]; cnt[d] = 2 * g.reach[d] + 1; }
-        }
-        ncol = cnt[1] * cnt[2];
-        for (int col = lane; col < kMaxRuns / 2; col += 64) {
-            int s0 = 0, l0 = 0, s1 = 0, l1 = 0;
-            if (col < ncol) {
-                const int wy = (lo[1] + col % cnt[1] + g.nc[1]) % g.nc[1];
-                const int wz = (lo[2] + col / cnt[1] + g.nc[2]) % g.nc[2];
-                const int base = (wz * g.nc[1] + wy) * g.nc[0];
-                int x0 = lo[0], x1 = lo[0] + cnt[0] - 1, a0 = 0, b0 = -1;
-                if (x0 < 0) { a0 = x0 + g.nc[0]; b0 = g.nc[0] - 1; x0 = 0; }
-                else if (x1 >= g.nc[0]) { a0 = 0; b0 = x1 - g.nc[0]; x1 = g.nc[0] - 1; }
-                s0 = g.cell_start[base + x0];
-                l0 = g.cell_start[base + x1] + g.cell_count[base + x1] - s0;
-                if (b0 >= a0) {
-                    s1 = g.cell_start[base + a0];
-                    l1 = g.cell_start[base + b0] + g.cell_count[base + b0] - s1;
-                }
-            }
-            run_start[w][2 * col] = s0; run_len[w][2 * col] = l0;
-            run_start[w][2 * col + 1] = s1; run_len[w][2 * col + 1] = l1;
-        }
+    const int cc[3] = {ci % g.nc[0], (ci / g.nc[0]) % g.nc[1], ci / (g.nc[0] * g.nc[1])};
+    int lo[3], cnt[3];
+    for (int d = 0; d < 3; d++) {
+        if (2 * g.reach[d] + 1 >= g.nc[d]) { lo[d] = 0; cnt[d] = g.nc[d]; }
+        else { lo[d] = cc[d] - g.reach[d]; cnt[d] = 2 * g.reach[d] + 1; }
     }
-    __syncthreads();
-
-    // phase 2: stream the runs, kUnroll loads in flight per lane; every hit
-    // sets bit j of the row bitmap
-    if (active) {
-        for (int r = 0; r < 2 * ncol; r++) {
-            const int rs = run_start[w][r], rl = run_len[w][r];
-            for (int off = 0; off < rl; off += 64 * kUnroll) {
-                double4 pj[kUnroll];
-#pragma unroll
-                for (int u = 0; u < kUnroll; u++) {
-                    const int t = off + u * 64 + lane;
-                    pj[u] = t < rl ? g.cpos[rs + t] : make_double4(0.0, 0.0, 0.0, -1.0);
-                }
-#pragma unroll
-                for (int u = 0; u < kUnroll; u++) {
-                    const int code = (int)pj[u].w;
-                    const int j = code >> 1;
-                    if (code >= 0 && j != i && !(fi && (code & 1)) && within<PBC>(s, pj[u], xi, yi, zi))
-                        atomicOr(&B[j >> 5], 1u << (j & 31));
-                }
+    const int ncol = cnt[1] * cnt[2];
+    if (t < kMaxRuns / 2) {
+        const int col = t;
+        int s0 = 0, l0 = 0, s1 = 0, l1 = 0;
+        if (col < ncol) {
+            const int wy = (lo[1] + col % cnt[1] + g.nc[1]) % g.nc[1];
+            const int wz = (lo[2] + col / cnt[1] + g.nc[2]) % g.nc[2];
+            const int base = (wz * g.nc[1] + wy) * g.nc[0];
+            int x0 = lo[0], x1 = lo[0] + cnt[0] - 1, a0 = 0, b0 = -1;
+            if (x0 < 0) { a0 = x0 + g.nc[0]; b0 = g.nc[0] - 1; x0 = 0; }
+            else if (x1 >= g.nc[0]) { a0 = 0; b0 = x1 - g.nc[0]; x1 = g.nc[0] - 1; }
+            s0 = g.cell_start[base + x0];
+            l0 = g.cell_start[base + x1] + g.cell_count[base + x1] - s0;
+            if (b0 >= a0) {
+                s1 = g.cell_start[base + a0];
+                l1 = g.cell_start[base + b0] + g.cell_count[base + b0] - s1;
             }
         }
+        run_start[2 * col] = s0; run_len[2 * col] = l0;
+        run_start[2 * col + 1] = s1; run_len[2 * col + 1] = l1;
     }
-    __syncthreads();
-    // phase 3: exclusions only remove pairs -- clear the bits of i's
-    // excluded partners
-    if (active) {
-        for (int k = s.xfull_off[i] + lane; k < s.xfull_off[i + 1]; k += 64) {
-            const int e = s.xfull_idx[k];
-            atomicAnd(&B[e >> 5], ~(1u << (e & 31)));
-        }
-    }
-    __syncthreads();
-    if (!active) return;
+    return ncol;
+}
 
-    // phase 4: ordered compaction of the bitmap (no barriers below)
+// Inclusive prefix sum over the 64 lanes of a wavefront: row_shr 1, 2, 4, 8
+// within rows of 16 lanes, then row_bcast 15 and 31 carry the row totals.
+__device__ __forceinline__ int wave_incl_scan(int v)
+{
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
+    return v;
+}
+
+// Ordered compaction of row i's bitmap B (LDS) into its ELL row; sets cnt,
+// half (entries below i) and the overflow high-water mark.  One wavefront,
+// no barriers.
+__device__ void compact_row(const DevSys &s, const uint32_t *B, int words, int i, int lane)
+{
     int *row = s.nbr + (size_t)i * s.cap;
     int total = 0, nlow = 0;
     const int iw = i >> 5;
@@ -201,32 +278,188 @@ __global__ __launch_bounds__(256) void k_build_rows(DevSys s, CellGrid g, int wo
         const int k = base + lane;
         uint32_t wd = k < words ? B[k] : 0u;
         const int pc = __popc(wd);
-        int incl = pc;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            int t = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += t;
-        }
-        const int tot = __shfl(incl, 63, 64);
+        const int incl = wave_incl_scan(pc);
         int pos = total + incl - pc;
-        int lowc = 0;
-        if (k < iw) lowc = pc;
-        else if (k == iw) lowc = __popc(wd & ((1u << (i & 31)) - 1u));
+        if (iw >= base && iw < base + 64) {
+            const int below = pos + __popc(wd & ((1u << (i & 31)) - 1u));
+            nlow = __builtin_amdgcn_readlane(below, iw - base);
+        }
         while (wd) {
             const int b = __ffs(wd) - 1;
             wd &= wd - 1u;
             if (pos < s.cap) row[pos] = k * 32 + b;
             pos++;
         }
-        for (int o = 32; o > 0; o >>= 1) lowc += __shfl_xor(lowc, o, 64);
-        nlow += lowc;
-        total += tot;
+        total += __builtin_amdgcn_readlane(incl, 63);
     }
     if (lane == 0) {
         s.cnt[i] = total < s.cap ? total : s.cap;
         s.half[i] = nlow < s.cap ? nlow : s.cap;
         if (total > s.cap) atomicMax(s.overflow, total);
     }
+}
+
+// Rows of a group of up to G atoms of one cell (group `blockIdx.x` belongs
+// to the last cell whose group_start is not above it), four wavefronts.
+// Per candidate j the G row tests set bits of a hit mask; a candidate with
+// some r^2 in the band around the list radius redoes its tests in FP64.
+// Self pairs (r^2 = 0) are cleared with the exclusions.
+template <int G, bool PBC>
+__global__ __launch_bounds__(64 * kRowWaves) void k_build_groups(DevSys s, CellGrid g, int words, float cut2,
+                                                                  float band, int center_ok)
+{
+    constexpr int NT = 64 * kRowWaves;
+    extern __shared__ uint32_t bitmap[];     // G rows of `words` words
+    __shared__ int run_start[kMaxRuns], run_len[kMaxRuns], run_pref[kMaxRuns + 1];
+    __shared__ int s_cell, s_p0, s_na;
+    __shared__ int s_a[G];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int grp = blockIdx.x;
+    if (grp >= g.group_start[g.ncell]) return;       // uniform over the block
+    if (tid == 0) {
+        int lo = 0, hi = g.ncell - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (g.group_start[mid] <= grp) lo = mid; else hi = mid - 1;
+        }
+        const int k = grp - g.group_start[lo];
+        s_cell = lo;
+        s_p0 = g.cell_start[lo] + k * G;
+        s_na = min(G, g.cell_count[lo] - k * G);
+    }
+    for (int k = tid; k < G * words; k += NT) bitmap[k] = 0u;
+    __syncthreads();
+    const int na = s_na, ci = s_cell;
+    if (tid < na) s_a[tid] = g.cell_atoms[s_p0 + tid];
+    const int nrun = 2 * cell_runs(g, ci, tid, run_start, run_len);
+    __syncthreads();
+    // the runs as one index space: prefix of their lengths (wavefront 0)
+    if (tid < 64) {
+        const int incl = wave_incl_scan(tid < nrun ? run_len[tid] : 0);
+        run_pref[tid + 1] = incl;
+        if (tid == 0) run_pref[0] = 0;
+    }
+    __syncthreads();
+    const int total = run_pref[nrun];
+
+    // row atoms: FP32 offsets from the cell centre (periodic), FP64 copies
+    // for the band test; absent rows sit far away
+    int4 pc = make_int4(0, 0, 0, 0);
+    if (PBC) {
+        const int cc[3] = {ci % g.nc[0], (ci / g.nc[0]) % g.nc[1], ci / (g.nc[0] * g.nc[1])};
+        pc.x = (int)(unsigned)(long long)((cc[0] + 0.5) / g.nc[0] * 4294967296.0);
+        pc.y = (int)(unsigned)(long long)((cc[1] + 0.5) / g.nc[1] * 4294967296.0);
+        pc.z = (int)(unsigned)(long long)((cc[2] + 0.5) / g.nc[2] * 4294967296.0);
+    }
+    const float ux = (float)s.unit[0], uy = (float)s.unit[1], uz = (float)s.unit[2];
+    int4 pi[G];
+    float ox[G], oy[G], oz[G];
+    unsigned fmask = 0u;
+#pragma unroll
+    for (int q = 0; q < G; q++) {
+        const int aa = q < na ? s_a[q] : s_a[0];
+        if (q < na && s.frozen[aa]) fmask |= 1u << q;
+        if (PBC) {
+            pi[q] = s.ipos[aa];
+            ox[q] = q < na ? (float)wdiff(pi[q].x, pc.x) * ux : 1e18f;
+            oy[q] = (float)wdiff(pi[q].y, pc.y) * uy;
+            oz[q] = (float)wdiff(pi[q].z, pc.z) * uz;
+        }
+    }
+    const unsigned live = na >= 32 ? ~0u : ((1u << na) - 1u);
+    int r = 0;                                   // run of this thread's next candidate
+    {
+        for (int base = 0; base < total; base += NT * kUnroll) {
+            int code[kUnroll];
+            int4 pj[kUnroll];
+            double4 pd[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) {
+                const int f = base + u * NT + tid;
+                code[u] = -1;
+                if (f < total) {
+                    while (run_pref[r + 1] <= f) r++;
+                    const int t = run_start[r] + f - run_pref[r];
+                    if (PBC) { pj[u] = g.cfix[t]; code[u] = pj[u].w; }
+                    else { pd[u] = g.cpos[t]; code[u] = (int)pd[u].w; }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) {
+                if (code[u] < 0) continue;
+                const int j = code[u] >> 1;
+                unsigned hits = 0u;
+                bool exact = !PBC;
+                if (PBC) {
+                    float dmin = 3.0e38f;
+                    if (center_ok) {
+                        const float cx = (float)wdiff(pj[u].x, pc.x) * ux;
+                        const float cy = (float)wdiff(pj[u].y, pc.y) * uy;
+                        const float cz = (float)wdiff(pj[u].z, pc.z) * uz;
+#pragma unroll
+                        for (int q = 0; q < G; q++) {
+                            const float dx = cx - ox[q], dy = cy - oy[q], dz = cz - oz[q];
+                            const float d = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) - cut2;
+                            dmin = fminf(dmin, fabsf(d));
+                            hits |= d < 0.f ? (1u << q) : 0u;
+                        }
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < G; q++) {
+                            const float dx = (float)wdiff(pj[u].x, pi[q].x) * ux;
+                            const float dy = (float)wdiff(pj[u].y, pi[q].y) * uy;
+                            const float dz = (float)wdiff(pj[u].z, pi[q].z) * uz;
+                            const float d = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) - cut2;
+                            dmin = fminf(dmin, fabsf(d));
+                            hits |= d < 0.f ? (1u << q) : 0u;
+                        }
+                    }
+                    exact = dmin <= band;
+                }
+                if (exact) {
+                    // FP64 test of every row (the reference's r2 and test)
+                    const double4 pjd = PBC ? make_double4(s.x[j], s.y[j], s.z[j], 0.0) : pd[u];
+                    hits = 0u;
+                    for (int q = 0; q < na; q++) {
+                        const int aq = s_a[q];
+                        if (within<PBC>(s, pjd, s.x[aq], s.y[aq], s.z[aq])) hits |= 1u << q;
+                    }
+                }
+                hits &= live;
+                if (code[u] & 1) hits &= ~fmask;
+                const uint32_t jb = 1u << (j & 31);
+                const int jw = j >> 5;
+#pragma unroll
+                for (int q = 0; q < G; q++)
+                    if (hits & (1u << q)) atomicOr(&bitmap[q * words + jw], jb);
+            }
+        }
+    }
+    __syncthreads();
+    // exclusions and the self pair only remove pairs
+    for (int q = 0; q < na; q++) {
+        const int aq = s_a[q];
+        if (tid == 0) atomicAnd(&bitmap[q * words + (aq >> 5)], ~(1u << (aq & 31)));
+        for (int k = s.xfull_off[aq] + tid; k < s.xfull_off[aq + 1]; k += NT) {
+            const int e = s.xfull_idx[k];
+            atomicAnd(&bitmap[q * words + (e >> 5)], ~(1u << (e & 31)));
+        }
+    }
+    __syncthreads();
+    for (int q = wv; q < na; q += kRowWaves) compact_row(s, bitmap + q * words, words, s_a[q], lane);
+}
+
+template <int G>
+void launch_groups(const DevSys &s, const CellGrid &g, int words, float cut2, float band, int center_ok,
+                   hipStream_t st)
+{
+    // groups: at most ceil(n / G) + ncell; surplus blocks return at once
+    const dim3 grid((s.n + G - 1) / G + g.ncell), block(64 * kRowWaves);
+    const size_t lds = (size_t)words * 4 * G;
+    if (s.pbtype != 0)
+        hipLaunchKernelGGL((k_build_groups<G, true>), grid, block, lds, st, s, g, words, cut2, band, center_ok);
+    else
+        hipLaunchKernelGGL((k_build_groups<G, false>), grid, block, lds, st, s, g, words, cut2, band, center_ok);
 }
 
 }  // namespace
@@ -239,21 +472,37 @@ size_t list_build_lds_bytes(int n)
 
 void launch_list_build(const DevSys &s, const CellGrid &g, hipStream_t st)
 {
-    const int nb = (s.n + 255) / 256;
-    hipLaunchKernelGGL(k_zero_int, dim3(64), dim3(256), 0, st, g.cell_count, g.ncell);
-    hipLaunchKernelGGL(k_zero_int, dim3(64), dim3(256), 0, st, g.cursor, g.ncell);
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, s.overflow, 1);
-    hipLaunchKernelGGL(k_cell_count, dim3(nb), dim3(256), 0, st, s, g);
-    hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, st, g);
-    hipLaunchKernelGGL(k_cell_fill, dim3(nb), dim3(256), 0, st, s, g);
     const int words = (s.n + 31) / 32;
-    const size_t per_wave = (size_t)words * 4;
-    int wpb = 4;
-    while (wpb > 1 && per_wave * wpb > 60 * 1024) wpb >>= 1;
-    const size_t lds = per_wave * wpb;
-    const int blocks = (s.n + wpb - 1) / wpb;
-    if (s.pbtype == 0)
-        hipLaunchKernelGGL(k_build_rows<false>, dim3(blocks), dim3(64 * wpb), lds, st, s, g, words, wpb);
-    else
-        hipLaunchKernelGGL(k_build_rows<true>, dim3(blocks), dim3(64 * wpb), lds, st, s, g, words, wpb);
+    const size_t per_row = (size_t)words * 4;
+    int G = 8;
+    while (G > 1 && per_row * G > 60 * 1024) G >>= 1;
+    if (g.ncell <= kBinCells) {
+        hipLaunchKernelGGL(k_bin, dim3(1), dim3(1024), 0, st, s, g, G);
+    } else {
+        const int nb = (s.n + 255) / 256;
+        hipLaunchKernelGGL(k_zero_int, dim3(64), dim3(256), 0, st, g.cell_count, g.ncell);
+        hipLaunchKernelGGL(k_zero_int, dim3(64), dim3(256), 0, st, g.cursor, g.ncell);
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, s.overflow, 1);
+        hipLaunchKernelGGL(k_cell_count, dim3(nb), dim3(256), 0, st, s, g);
+        hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, st, g, G);
+        hipLaunchKernelGGL(k_cell_fill, dim3(nb), dim3(256), 0, st, s, g);
+    }
+    if (g.ncell > 1) hipLaunchKernelGGL(k_cell_sort, dim3(g.ncell), dim3(256), 0, st, s, g);
+    // the FP32 test relative to the cell centre needs the minimum image
+    // about the centre to be the one about every atom of the cell for all
+    // pairs up to the list radius: L/2 > (ctoff + dcut) + half a cell
+    // diagonal (+ 1e-3 A) on every axis
+    int center_ok = s.pbtype != 0;
+    double diag2 = 0.0;
+    for (int d = 0; d < 3; d++) diag2 += (s.L[d] / g.nc[d]) * (s.L[d] / g.nc[d]);
+    const double rcut = std::sqrt(s.cutnb2);
+    for (int d = 0; d < 3; d++)
+        if (!(0.5 * s.L[d] > rcut + 0.5 * std::sqrt(diag2) + 1e-3)) center_ok = 0;
+    const float cut2 = (float)s.cutnb2, band = (float)(s.cutnb2 * kBand);
+    switch (G) {
+    case 8: launch_groups<8>(s, g, words, cut2, band, center_ok, st); break;
+    case 4: launch_groups<4>(s, g, words, cut2, band, center_ok, st); break;
+    case 2: launch_groups<2>(s, g, words, cut2, band, center_ok, st); break;
+    default: launch_groups<1>(s, g, words, cut2, band, center_ok, st); break;
+    }
 }

@@ -216,7 +216,13 @@ void setup_grid(mcx_ctx *h)
     h->cell_count.alloc(g.ncell);
     h->cursor.alloc(g.ncell);
     h->cpos.alloc(h->n);
+    h->cfix.alloc(h->n);
+    h->cell_tmp.alloc(h->n);
+    h->group_start.alloc(g.ncell + 1);
     g.cpos = h->cpos.p;
+    g.cfix = h->cfix.p;
+    g.cell_tmp = h->cell_tmp.p;
+    g.group_start = h->group_start.p;
     g.cell_of = h->cell_of.p;
     g.cell_atoms = h->cell_atoms.p;
     g.cell_start = h->cell_start.p;

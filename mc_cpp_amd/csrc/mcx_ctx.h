@@ -76,7 +76,8 @@ struct mcx_ctx {
     DBuf<double4> cpos;
     DBuf<int> nbr, cnt, half, overflow;
     int cap = 0;
-    DBuf<int> cell_of, cell_start, cell_count, cell_atoms, cursor;
+    DBuf<int> cell_of, cell_start, cell_count, cell_atoms, cursor, group_start, cell_tmp;
+    DBuf<int4> cfix;
     CellGrid grid{};
     DBuf<int2> tiles;
     int ntiles = 0;
@@ -153,12 +154,13 @@ struct mcx_ctx {
         for (auto *b : d) b->release();
         ipos.release(); es.release(); esk.release(); es14.release(); qes.release();
         cpos.release();
+        cfix.release();
         adj_ids.release();
         adj_par.release();
         adj_type.release();
         mvb.release();
         DBuf<int> *ib[] = {&frozen_d, &excl_off_d, &excl_idx_d, &xfull_off_d, &xfull_idx_d, &nbr, &cnt, &half, &overflow, &cell_of,
-                           &cell_start, &cell_count, &cell_atoms, &cursor, &adj_off, &adj_kk};
+                           &cell_start, &cell_count, &cell_atoms, &cursor, &group_start, &cell_tmp, &adj_off, &adj_kk};
         for (auto *b : ib) b->release();
         tiles.release();
         for (int k = 0; k < kNumKinds; k++) { t_idx[k].release(); t_type[k].release(); t_par[k].release(); }

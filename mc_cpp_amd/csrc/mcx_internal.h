@@ -146,6 +146,9 @@ struct CellGrid {
     int reach[3];            // cells searched in each direction
     int *cell_of, *cell_start, *cell_count, *cell_atoms, *cursor;
     double4 *cpos;           // cell-ordered (x, y, z, 2*atom + frozen)
+    int4 *cfix;              // cell-ordered (fixed-point x, y, z, 2*atom + frozen)
+    int *cell_tmp;           // binned atoms before the in-cell sort
+    int *group_start;        // [ncell + 1] first row group of each cell (periodic build)
 };
 void launch_list_build(const DevSys &s, const CellGrid &g, hipStream_t st);
 size_t list_build_lds_bytes(int n);

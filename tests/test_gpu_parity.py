@@ -110,6 +110,32 @@ def test_list_after_coordinate_update_and_boundaries():
     _check_list(ff.h, o)
 
 
+@pytest.mark.parametrize("box", ["cubic", "orthorhombic"])
+def test_list_pairs_at_the_list_radius(box):
+    """Pairs placed at (ctoff + dcut) * (1 + e) for e from -4e-6 to +4e-6
+    (inside and outside the band the FP32 candidate test hands to the FP64
+    test) and at a few ulps from it; the list must equal the oracle's."""
+    s, _ = systems.argon_fcc(seed=12)
+    if box == "orthorhombic":
+        s.pbtype = systems.PBC_ORBIC
+        s.box = np.array([34.39, 36.1, 33.2])
+    rc = s.ctoff + s.dcut
+    rng = np.random.default_rng(17)
+    rel = [-4e-6, -3.9e-6, -1e-6, -1e-7, -1e-12, -1e-15, 0.0, 1e-15, 1e-12, 1e-7, 1e-6, 3.9e-6, 4e-6, 5e-6]
+    for k in range(200):
+        a, b = 4 * k, 4 * k + 1
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        d = rc * (1.0 + rel[k % len(rel)]) * u
+        pos = np.array([s.x[a], s.y[a], s.z[a]]) + d
+        pos -= s.box * np.trunc(pos / s.box + np.where(pos >= 0, 0.5, -0.5))
+        s.x[b], s.y[b], s.z[b] = pos
+    o, ref = _pair(s)
+    o.update_list()
+    ff = mcx.FField(s)
+    _check_list(ff.h, o)
+
+
 def test_frozen_pairs_excluded():
     s, _ = systems.argon_fcc(seed=4)
     s.frozen[: s.natom // 2] = 1
