@@ -43,6 +43,7 @@ sys.path.insert(0, ROOT)
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 FLOP_FULL_PAIR = 32.0          # per pair, FMA = 2 (the constant frozen in csrc/mcx_internal.h)
+COPY_BYTES = 256 << 20         # calibration copy of the counter runs
 
 
 def cpu_reference_baseline(s, fp, nmoves, seed):
@@ -134,6 +135,43 @@ def force_bias_vs_translation(args):
     return out
 
 
+def kernel_probe():
+    """Fixed kernel sequence for the HBM-traffic counter runs
+    (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE): 20 launches of the O(N^2)
+    pair kernel on the 12,288-atom box, then 5 launches of a plain device
+    copy of COPY_BYTES (torch's elementwise copy of float32), whose counter
+    readings calibrate the counters' units."""
+    import torch
+    from mc_cpp_amd import mcx, systems
+    s, _ = systems.water_box(seed=2)
+    s.cutmode = systems.CUT_FULL
+    h = mcx.from_system(s)
+    h.energy()
+    h.time_kernel(0, 20)
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": "k_full_energy x21 (1 warm-up), copy x5", "copy_bytes": COPY_BYTES,
+                      "atoms": s.natom}), flush=True)
+
+
+def traffic_from_profile():
+    """HBM bytes per launch of k_full_energy from the committed counter
+    summary (a line 'traffic_bytes_per_launch: <int>'), or None."""
+    path = os.path.join(ROOT, "profiles", "k_full_energy_water12288_pmc.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,7 +186,11 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--fb-moves", type=int, default=20000, help="moves per force-bias / translation run")
     ap.add_argument("--no-fbias", action="store_true")
+    ap.add_argument("--probe", action="store_true", help="kernel sequence for the HBM counter runs only")
     args = ap.parse_args()
+    if args.probe:
+        kernel_probe()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the Metropolis chain is sequential and this build has no "
               "multi-GPU path; run with --gpus 1", file=sys.stderr)
@@ -237,7 +279,9 @@ def main():
         "switch_energy_ms": ms_sw,
         "roofline": {"bound": "valu", "kernel": "k_full_energy (O(N^2) min-image LJ+Coulomb, 12,288 atoms)",
                      "achieved": tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": tflops / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": tflops / PEAK_FP32_TFLOPS, "traffic": traffic_from_profile(),
+                     "peak_without_packed_fp32": PEAK_FP32_TFLOPS / 2,
+                     "frac_without_packed_fp32": tflops / (PEAK_FP32_TFLOPS / 2),
                      "flop_per_launch": flop, "ms_per_launch": ms_full},
         "list_build": {"bound": "hbm", "achieved": list_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                        "frac": list_gbs / PEAK_HBM_GBS, "bytes_per_launch": list_bytes, "ms_per_launch": ms_list,
