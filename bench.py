@@ -15,6 +15,10 @@ replayed as a hipGraph, launch shape picked by the library's tuner; compared
 with the same replicas run one at a time through the single-chain device
 loop -> "batched".
 
+Large mixed system (same run): the ~100k-atom water + chain box -- pair
+kernels, list build, and the device loop with the 50-move list schedule
+and with displacement-triggered rebuilds -> "large_mixed".
+
 Force-bias (same run): residue force-bias moves (A = 0.01 A^2) against
 plain residue translations on the 12,288-atom box, moves/s and acceptance;
 the adjoint kernel's time on that box and a 64-configuration batched
@@ -135,6 +139,36 @@ def force_bias_vs_translation(args):
     return out
 
 
+def large_mixed(args):
+    """The ~100k-atom mixed system (32,768-site water lattice minus the
+    waters within 2.6 A of a 2,000-bead chain, L = 99.4 A): pair kernels,
+    list build and the device Metropolis loop with the reference's 50-move
+    list schedule and with displacement-triggered rebuilds."""
+    from mc_cpp_amd import mcx, systems
+    s, _ = systems.mixed_system()
+    n = s.natom
+    h = mcx.FField(s).h
+    npairs = h.update_list()
+    out = {"workload": "mixed_%d: water lattice + 2,000-bead chain, L=99.4 A, switch 10/12 A, skin 2 A, T=300 K, "
+                       "residue TRN 0.3 A / ROT 15 deg (waters) + atom TRN 0.1 A (chain)" % n,
+           "atoms": n, "list_pairs": npairs,
+           "switch_energy_ms": h.time_kernel(1, 20), "list_build_ms": h.time_kernel(2, 5)}
+    sf = s.copy()
+    sf.cutmode = systems.CUT_FULL
+    hf = mcx.from_system(sf)
+    hf.energy()
+    ms = hf.time_kernel(0, 3)
+    out["full_energy_ms"] = ms
+    out["full_pair_evals_per_s"] = n * (n - 1) / 2 / (ms * 1e-3)
+    for name, every in (("mc_list_every_50", 50), ("mc_list_auto", mcx.LIST_AUTO)):
+        h.mc_run(s.moves, 500, 1, rng_mode=mcx.MC_PHILOX, list_every=every, record=False)
+        r = h.mc_run(s.moves, args.large_moves, 2, rng_mode=mcx.MC_PHILOX, list_every=every, record=False)
+        out[name] = {"moves_per_s": args.large_moves / r["seconds"], "moves": args.large_moves,
+                     "acceptance": float(r["accepted"].sum()) / args.large_moves,
+                     "list_rebuilds": int(r["list_updates"])}
+    return out
+
+
 def kernel_probe():
     """Fixed kernel sequence for the HBM-traffic counter runs
     (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE): 20 launches of the O(N^2)
@@ -187,6 +221,8 @@ def main():
     ap.add_argument("--fb-moves", type=int, default=20000, help="moves per force-bias / translation run")
     ap.add_argument("--no-fbias", action="store_true")
     ap.add_argument("--probe", action="store_true", help="kernel sequence for the HBM counter runs only")
+    ap.add_argument("--large-moves", type=int, default=10000, help="moves per run on the ~100k-atom system")
+    ap.add_argument("--no-large", action="store_true")
     args = ap.parse_args()
     if args.probe:
         kernel_probe()
@@ -245,6 +281,7 @@ def main():
 
     batch = None if args.no_batch else batched_replicas(args)
     fbias = None if args.no_fbias else force_bias_vs_translation(args)
+    large = None if args.no_large else large_mixed(args)
 
     cpu = None
     if not args.no_cpu:
@@ -288,6 +325,7 @@ def main():
                        "pairs": npairs},
         "batched": batch,
         "force_bias": fbias,
+        "large_mixed": large,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

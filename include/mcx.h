@@ -148,6 +148,7 @@ typedef struct mcx_mc_stats {
     double seconds;        /* wall time of the device loop */
 } mcx_mc_stats;
 
+#define MCX_LIST_AUTO  (-1) /* list_every: rebuild when an accepted move carries an atom beyond half the skin */
 #define MCX_MC_REPLAY   1  /* draw from std::mt19937(seed) + generate_canonical exactly as src/MC.cpp:119-156 */
 #define MCX_MC_PHILOX   0  /* counter-based Philox4x32-10 proposals (no replay) */
 
@@ -155,8 +156,12 @@ typedef struct mcx_mc_stats {
  * evaluated on the device and the Metropolis test of apply_criterion
  * (:175-204), including its beta = 1/((0.83144621/418.4)*T) applied to dE
  * in iu.  The list is rebuilt every `list_every` steps (the reference uses
- * 50, :151-154).  Optional per-step outputs (nsteps entries each): accepted
- * flag, dE, move type, move instance. */
+ * 50, :151-154).  list_every = MCX_LIST_AUTO (periodic boxes; not in the
+ * reference) rebuilds instead after any accepted move that carries an atom
+ * farther than dcut/2 from its position at the last build, and evaluates a
+ * trial that does so over every atom: every dE then sums exactly the pairs
+ * inside the cutoff.  Optional per-step outputs (nsteps entries each):
+ * accepted flag, dE, move type, move instance. */
 int mcx_mc_run(mcx_ctx *h, const mcx_move_type *types, int ntypes, int nsteps, uint64_t seed,
                int rng_mode, int list_every, mcx_mc_stats *stats, unsigned char *acc_out,
                double *de_out, int *mvtyp_out, int *mvinst_out);

@@ -141,6 +141,7 @@ __global__ __launch_bounds__(1024) void k_bin(DevSys s, CellGrid g, int group)
     for (int a = threadIdx.x; a < s.n; a += 1024) {
         const int c = cell_of_atom(s, g, a);
         g.cell_of[a] = c;
+        g.ipos_ref[a] = s.ipos[a];
         atomicAdd(&cnt[c], 1);
     }
     __syncthreads();
@@ -163,6 +164,7 @@ __global__ void k_cell_count(DevSys s, CellGrid g)
     if (a >= s.n) return;
     const int c = cell_of_atom(s, g, a);
     g.cell_of[a] = c;
+    g.ipos_ref[a] = s.ipos[a];
     atomicAdd(&g.cell_count[c], 1);
 }
 

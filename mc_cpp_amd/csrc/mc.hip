@@ -29,6 +29,8 @@ constexpr int kMCW = kMCT / 64;
 // Shared state of one move.
 struct MoveLDS {
     int first, last, nm, total, acc, type, inst, pivot;
+    int scan;                    // this trial leaves the list's validity: every atom is a candidate
+    int stop;                    // an accepted move left it: end the chunk for a rebuild
     double r[3], R[3][3], piv[3];
     double tx[kMaxMoveAtoms], ty[kMaxMoveAtoms], tz[kMaxMoveAtoms];
     double dx[kMaxMoveAtoms], dy[kMaxMoveAtoms], dz[kMaxMoveAtoms];   // unwrapped displacement of each moved atom
@@ -150,6 +152,7 @@ __device__ void move_delta(const DevSys &s, const DevTerms &t, const DevMoves &m
         constexpr int NP = kMCT - 64;
         const int pt = tid - 64;
         const int total = m.total, nm = m.nm;
+        const bool all = s.cutmode == 0 || m.scan;
         for (int base = 0; base < total; base += kPairU * NP) {
             int av[kPairU], jv[kPairU];
 #pragma unroll
@@ -165,7 +168,7 @@ __device__ void move_delta(const DevSys &s, const DevTerms &t, const DevMoves &m
                     }
                     const int k = e - m.pref[lo];
                     av[u] = lo;
-                    jv[u] = s.cutmode == 0 ? k : s.nbr[(size_t)(m.first + lo) * s.cap + k];
+                    jv[u] = all ? k : s.nbr[(size_t)(m.first + lo) * s.cap + k];
                 }
             }
             int4 pj[kPairU];
@@ -179,7 +182,8 @@ __device__ void move_delta(const DevSys &s, const DevTerms &t, const DevMoves &m
                 const int a = av[u], j = jv[u];
                 if (a < 0) continue;
                 const int i = m.first + a;
-                if (s.cutmode == 0 && (j == i || excluded(s, i, j))) continue;
+                if (all && (j == i || excluded(s, i, j))) continue;
+                if (m.scan && s.frozen[i] && s.frozen[j]) continue;   // not a list pair
                 int b = -1;
                 if (j >= m.first && j <= m.last) {
                     if (j < i) continue;     // pair of two moved atoms: counted once
@@ -267,6 +271,7 @@ __device__ void load_moved(const DevSys &s, const DevMoves &mv, MoveLDS &m)
         m.mc[a] = s.cutmode == 0 ? s.n : s.cnt[i];
         m.ms[a] = mv.mass ? mv.mass[i] : 1.0;
     } else if (a == kMaxMoveAtoms) {
+        m.scan = 0;
         const int pv = m.pivot;
         if (pv >= 0 && (pv < m.first || pv > m.last)) {
             m.xpiv[0] = s.x[pv]; m.xpiv[1] = s.y[pv]; m.xpiv[2] = s.z[pv];
@@ -283,9 +288,19 @@ __device__ void load_moved(const DevSys &s, const DevMoves &mv, MoveLDS &m)
     __syncthreads();
 }
 
+// Is fixed-point position p farther than half the list skin from its
+// position r at the last list build?
+__device__ __forceinline__ bool beyond_half_skin(const DevSys &s, int4 p, int4 r, double half_skin2)
+{
+    const double dx = (double)mcx_wdiff(p.x, r.x) * s.unit[0];
+    const double dy = (double)mcx_wdiff(p.y, r.y) * s.unit[1];
+    const double dz = (double)mcx_wdiff(p.z, r.z) * s.unit[2];
+    return dx * dx + dy * dy + dz * dz > half_skin2;
+}
+
 // Phase C: trial coordinates (MOVE_TRN::tr_a_b / MOVE_ROT::apply_rotation,
 // include/Move.hpp:42-50,120-143, each atom re-wrapped) and displacements.
-__device__ void make_trial(const DevSys &s, MoveLDS &m)
+__device__ void make_trial(const DevSys &s, MoveLDS &m, const int4 *ipos_ref, double half_skin2)
 {
     const int a = threadIdx.x;
     if (a < m.nm) {
@@ -315,15 +330,20 @@ __device__ void make_trial(const DevSys &s, MoveLDS &m)
         }
         m.dx[a] = ux - x; m.dy[a] = uy - y; m.dz[a] = uz - z;
         m.tx[a] = wrap1(s, ux, 0); m.ty[a] = wrap1(s, uy, 1); m.tz[a] = wrap1(s, uz, 2);
+        if (ipos_ref && beyond_half_skin(s, mcx_fixed4(s, m.tx[a], m.ty[a], m.tz[a], 0.f), ipos_ref[m.first + a],
+                                         half_skin2))
+            atomicOr(&m.scan, 1);
     }
     __syncthreads();
 }
 
 // Prefix of the moved atoms' list-row lengths (the flattened pair index of
 // move_delta and force_old).
-__device__ void row_prefix(MoveLDS &m)
+__device__ void row_prefix(const DevSys &s, MoveLDS &m)
 {
     if (threadIdx.x == 64) {
+        if (m.scan)
+            for (int b = 0; b < m.nm; b++) m.mc[b] = s.n;
         int acc = 0;
         for (int b = 0; b < m.nm; b++) { m.pref[b] = acc; acc += m.mc[b]; }
         m.pref[m.nm] = acc;
@@ -358,16 +378,19 @@ __global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s, DevTerms t, DevMove
     }
     if (arg.rng_mode == 1)
         for (int k = tid; k < kMTN; k += kMCT) mt[k] = arg.mt_state[k];
+    if (tid == 0) m.stop = 0;
     __syncthreads();
 
-    for (int step = 0; step < arg.nsteps; step++) {
+    int step = 0;
+    for (; step < arg.nsteps; step++) {
         const int st = arg.step0 + step + 1;
         // ---- phase A (wavefront 0, lanes in step): pick and draw the move
         // (src/MC_metropolis.cpp:85-116; src/MC.cpp:137-156)
-        if (w0) pick_move(mv, dr, dmax, m, lane);
+        if (w0 && !m.stop) pick_move(mv, dr, dmax, m, lane);
         __syncthreads();
+        if (m.stop) break;
         load_moved(s, mv, m);
-        row_prefix(m);
+        row_prefix(s, m);
         if (m.type == MCX_MV_FBIAS_DEV) {
             // force-bias proposal: dx = beta A F + sqrt(2A) xi with F the net
             // nonbonded force on the set in kcal/mol/A (iu/A / 418.4), A =
@@ -382,7 +405,8 @@ __global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s, DevTerms t, DevMove
             }
             __syncthreads();
         }
-        make_trial(s, m);
+        make_trial(s, m, arg.ipos_ref, arg.half_skin2);
+        if (m.scan) row_prefix(s, m);     // every atom is a candidate of this trial
         move_delta(s, t, mv, m);
         // ---- phase E+F (wavefront 0): Metropolis test (apply_criterion,
         // src/MC_metropolis.cpp:175-204), bookkeeping, commit of the moved
@@ -427,12 +451,20 @@ __global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s, DevTerms t, DevMove
                 }
                 adapt_dmax(mv, st, tp, acc, dmax, acc_tmp);
             }
+            bool far = false;
             if (acc && lane < m.nm) {
                 const int i = m.first + lane;
                 s.x[i] = m.tx[lane]; s.y[i] = m.ty[lane]; s.z[i] = m.tz[lane];
-                s.ipos[i] = mcx_fixed4(s, m.tx[lane], m.ty[lane], m.tz[lane], __int_as_float(m.ip[lane].w));
+                const int4 np = mcx_fixed4(s, m.tx[lane], m.ty[lane], m.tz[lane], __int_as_float(m.ip[lane].w));
+                s.ipos[i] = np;
+                far = arg.ipos_ref && beyond_half_skin(s, np, arg.ipos_ref[i], arg.half_skin2);
             }
+            if (__ballot(far) != 0ull && lane == 0) m.stop = 1;
         }
+    }
+    if (arg.stop_at && tid == 0) {
+        arg.stop_at[0] = step;
+        arg.stop_at[1] = m.stop;
     }
     __syncthreads();
     if (w0) {
@@ -454,7 +486,7 @@ __global__ __launch_bounds__(kMCT) void k_delta(DevSys s, DevTerms t, DevMoves m
     __shared__ MoveLDS m;
     const int nm = last - first + 1;
     if (threadIdx.x == 0) {
-        m.first = first; m.last = last; m.nm = nm; m.pivot = -1; m.type = MCX_MV_TRN_DEV;
+        m.first = first; m.last = last; m.nm = nm; m.pivot = -1; m.type = MCX_MV_TRN_DEV; m.scan = 0;
         m.ra0 = 0; m.ra1 = -1;
     }
     __syncthreads();

@@ -217,10 +217,12 @@ void setup_grid(mcx_ctx *h)
     h->cursor.alloc(g.ncell);
     h->cpos.alloc(h->n);
     h->cfix.alloc(h->n);
+    h->ipos_ref.alloc(h->n);
     h->cell_tmp.alloc(h->n);
     h->group_start.alloc(g.ncell + 1);
     g.cpos = h->cpos.p;
     g.cfix = h->cfix.p;
+    g.ipos_ref = h->ipos_ref.p;
     g.cell_tmp = h->cell_tmp.p;
     g.group_start = h->group_start.p;
     g.cell_of = h->cell_of.p;
@@ -872,6 +874,10 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
     return guarded(h, [&] {
         if (nsteps < 0) throw mcx_error(MCX_ERR_ARG, "nsteps < 0");
         if (!h->have_excl || !h->have_coords || !h->have_atoms) throw mcx_error(MCX_ERR_STATE, "system not set up");
+        const bool autolist = list_every == MCX_LIST_AUTO && h->cutmode != MCX_CUT_FULL;
+        if (list_every < 0 && list_every != MCX_LIST_AUTO) throw mcx_error(MCX_ERR_ARG, "list_every < 0");
+        if (autolist && h->pbtype == MCX_PBC_NONE)
+            throw mcx_error(MCX_ERR_UNSUPPORTED, "displacement-triggered lists need a periodic box");
         if (h->cutmode != MCX_CUT_FULL && !h->list_valid) build_list(h);
         DevMoves mv = build_moves(h, types, ntypes, h->mvb, kMaxMoveAtoms);
         // device-side run state
@@ -883,13 +889,14 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
         DBuf<double> de_d;
         DBuf<int> typ_d, inst_d;
         DBuf<double> disp_d, lnq_d;
+        DBuf<int> stop_d;
         struct Cleanup {
             std::vector<std::function<void()>> f;
             ~Cleanup() { for (auto &g : f) g(); }
         } cleanup;
         cleanup.f.push_back([&] { etot.release(); dmax.release(); mts.release(); trials.release(); accepted.release();
                                   acc_tmp.release(); acc_d.release(); de_d.release(); typ_d.release(); inst_d.release();
-                                  disp_d.release(); lnq_d.release(); });
+                                  disp_d.release(); lnq_d.release(); stop_d.release(); });
         double ener[10], r9[9];
         energy_device(h, r9);
         assemble(r9, ener);
@@ -926,12 +933,37 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
         a.mvinst_out = mvinst_out ? inst_d.p : nullptr;
         a.disp_out = disp_out ? disp_d.p : nullptr;
         a.lnq_out = lnq_out ? lnq_d.p : nullptr;
+        if (autolist) {
+            stop_d.alloc(2);
+            a.ipos_ref = h->ipos_ref.p;
+            a.half_skin2 = 0.25 * h->dcut * h->dcut;
+            a.stop_at = stop_d.p;
+        }
         HIPCHK(hipStreamSynchronize(h->stream));
         const auto t0 = std::chrono::steady_clock::now();
         long long nlist = 0;
         int done = 0;
+        // displacement-triggered list: launches of up to kAutoChunk moves,
+        // each ending early after an accepted move that carried an atom
+        // beyond half the skin; the list is then rebuilt
+        while (autolist && done < nsteps) {
+            constexpr int kAutoChunk = 4096;
+            a.nsteps = std::min(kAutoChunk, nsteps - done);
+            a.step0 = done;
+            launch_mc_chunk(h->sys(), h->dterms(), mv, a, h->stream);
+            HIPCHK(hipGetLastError());
+            int st[2] = {0, 0};
+            HIPCHK(hipMemcpyAsync(st, stop_d.p, sizeof(st), hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+            if (st[0] < 1 && !st[1]) throw mcx_error(MCX_ERR_HIP, "Metropolis chunk made no progress");
+            done += st[0];
+            if (st[1]) {
+                rebuild_list_fast(h);
+                nlist++;
+            }
+        }
         const int every = (list_every > 0 && h->cutmode != MCX_CUT_FULL) ? list_every : nsteps;
-        while (done < nsteps) {
+        while (!autolist && done < nsteps) {
             int chunk = every - (done % std::max(every, 1));
             if (chunk > nsteps - done) chunk = nsteps - done;
             a.nsteps = chunk;

@@ -77,7 +77,7 @@ struct mcx_ctx {
     DBuf<int> nbr, cnt, half, overflow;
     int cap = 0;
     DBuf<int> cell_of, cell_start, cell_count, cell_atoms, cursor, group_start, cell_tmp;
-    DBuf<int4> cfix;
+    DBuf<int4> cfix, ipos_ref;
     CellGrid grid{};
     DBuf<int2> tiles;
     int ntiles = 0;
@@ -155,6 +155,7 @@ struct mcx_ctx {
         ipos.release(); es.release(); esk.release(); es14.release(); qes.release();
         cpos.release();
         cfix.release();
+        ipos_ref.release();
         adj_ids.release();
         adj_par.release();
         adj_type.release();

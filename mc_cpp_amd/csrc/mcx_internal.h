@@ -147,6 +147,7 @@ struct CellGrid {
     int *cell_of, *cell_start, *cell_count, *cell_atoms, *cursor;
     double4 *cpos;           // cell-ordered (x, y, z, 2*atom + frozen)
     int4 *cfix;              // cell-ordered (fixed-point x, y, z, 2*atom + frozen)
+    int4 *ipos_ref;          // fixed-point positions at this build (displacement test)
     int *cell_tmp;           // binned atoms before the in-cell sort
     int *group_start;        // [ncell + 1] first row group of each cell (periodic build)
 };
@@ -172,6 +173,12 @@ struct MCRunArgs {
     unsigned char *acc_out; double *de_out; int *mvtyp_out; int *mvinst_out;  // device or null
     double *disp_out;      // [3*nsteps] translation of TRN and force-bias moves, or null
     double *lnq_out;       // [nsteps] log proposal-density ratio of force-bias moves, or null
+    // displacement-triggered list (auto mode): positions at the last build;
+    // a trial farther than half the skin from them is evaluated over every
+    // atom, and an accepted one ends the chunk (moves done -> *stop_at)
+    const int4 *ipos_ref;  // null: fixed rebuild schedule
+    double half_skin2;     // (dcut/2)^2, A^2
+    int *stop_at;          // device scalar
 };
 void launch_mc_chunk(const DevSys &s, const DevTerms &t, const DevMoves &mv, const MCRunArgs &a,
                      hipStream_t st);

@@ -27,6 +27,7 @@ MCX_OK = 0
 ERRORS = {1: "MCX_ERR_ARG", 2: "MCX_ERR_HIP", 3: "MCX_ERR_LIST_OVERFLOW", 4: "MCX_ERR_BOX_TOO_SMALL",
           5: "MCX_ERR_STATE", 6: "MCX_ERR_UNSUPPORTED"}
 MC_REPLAY, MC_PHILOX = 1, 0
+LIST_AUTO = -1      # list_every: displacement-triggered rebuilds (include/mcx.h MCX_LIST_AUTO)
 MAX_MOVE_TYPES = 8
 
 _dp = C.POINTER(C.c_double)

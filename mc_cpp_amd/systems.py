@@ -456,7 +456,8 @@ def mixed_system(seed: int = 3, nside: int = 32, L: float = 99.4, nbead: int = 2
     ww, wfp = water_box(seed=seed, nside=nside, L=L, tag=31)
     # chain by a seeded self-avoiding random walk: fixed bond length and
     # angle, torsion drawn uniformly, retried (next draws) while the new bead
-    # comes closer than 3.5 A to any earlier non-bonded bead
+    # comes closer than 3.5 A to any earlier non-bonded bead or its periodic
+    # image (the walk is longer than the box)
     u = philox.uniform(seed, 32, 64 * nbead)
     pos = np.zeros((nbead, 3))
     th0 = np.radians(111.0)
@@ -468,12 +469,16 @@ def mixed_system(seed: int = 3, nside: int = 32, L: float = 99.4, nbead: int = 2
         tmp = np.array([1.0, 0, 0]) if abs(b[0]) < 0.9 else np.array([0, 1.0, 0])
         p1 = np.cross(b, tmp); p1 /= np.linalg.norm(p1)
         p2 = np.cross(b, p1)
-        for attempt in range(48):
+        for attempt in range(64):
             phi = 2 * np.pi * u[cur % len(u)]
             cur += 1
             d = -np.cos(th0) * b + np.sin(th0) * (np.cos(phi) * p1 + np.sin(phi) * p2)
             cand = pos[i - 1] + 1.53 * d
-            if i < 3 or np.min(np.sum((pos[:i - 2] - cand) ** 2, 1)) > 3.5 ** 2:
+            if i < 3:
+                break
+            dd = pos[:i - 2] - cand
+            dd -= L * np.round(dd / L)
+            if np.min(np.sum(dd * dd, 1)) > 3.5 ** 2:
                 break
         pos[i] = cand
     pos -= pos.mean(0)
