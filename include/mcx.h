@@ -174,6 +174,14 @@ typedef struct mcx_mc_trace {
     int *mvinst;           /* [nsteps] move instance */
     double *disp;          /* [3*nsteps] translation (A) of TRN and FBIAS moves, 0 for ROT */
     double *lnq;           /* [nsteps] ln q(new->old)/q(old->new) of FBIAS moves, 0 otherwise */
+    /* Frames, as MC_metropolis::run writes tr.xyz / ener.dat every save_each
+     * steps (src/MC_metropolis.cpp:157-163): when save_each > 0 and on_frame
+     * is set, on_frame(user, step, etot, x, y, z) is called after every step
+     * that is a multiple of save_each with the running total (iu) and the
+     * coordinates (host arrays, valid during the call). */
+    int save_each;
+    void (*on_frame)(void *user, int step, double etot, const double *x, const double *y, const double *z);
+    void *user;
 } mcx_mc_trace;
 
 /* mcx_mc_run with the trace records above; mcx_mc_run is this call with

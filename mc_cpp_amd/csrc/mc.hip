@@ -21,8 +21,14 @@
 
 namespace {
 
-constexpr int kMCT = 1024;
-constexpr int kPairU = 4;          // list entries in flight per thread
+#ifndef MCX_MC_THREADS
+#define MCX_MC_THREADS 1024
+#endif
+#ifndef MCX_MC_UNROLL
+#define MCX_MC_UNROLL 4
+#endif
+constexpr int kMCT = MCX_MC_THREADS;
+constexpr int kPairU = MCX_MC_UNROLL;   // list entries in flight per thread
 constexpr int kMaxRec = 96;        // term records of one move staged in LDS
 constexpr int kMCW = kMCT / 64;
 

@@ -294,7 +294,7 @@ void rebuild_list_fast(mcx_ctx *h)
 
 void setup_tiles(mcx_ctx *h)
 {
-    const int nt = (h->n + 255) / 256;
+    const int nt = (h->n + kFullTile - 1) / kFullTile;
     std::vector<int2> t;
     for (int I = 0; I < nt; I++)
         for (int J = I; J < nt; J++) t.push_back(make_int2(I, J));
@@ -314,7 +314,7 @@ void energy_device(mcx_ctx *h, double out9[9])
         if (h->ntiles == 0) setup_tiles(h);
         nb_nb = h->ntiles;
         h->partial.alloc((size_t)std::max(nb_nb * 2, bonded_blocks(t) * 7));
-        launch_full_energy(s, h->tiles.p, h->ntiles, 256, h->partial.p, h->stream);
+        launch_full_energy(s, h->tiles.p, h->ntiles, kFullTile, h->partial.p, h->stream);
     } else {
         if (!h->list_valid) build_list(h);
         s = h->sys();
@@ -859,7 +859,7 @@ int mcx_mc_run(mcx_ctx *h, const mcx_move_type *types, int ntypes, int nsteps, u
                int list_every, mcx_mc_stats *stats, unsigned char *acc_out, double *de_out, int *mvtyp_out,
                int *mvinst_out)
 {
-    mcx_mc_trace tr{acc_out, de_out, mvtyp_out, mvinst_out, nullptr, nullptr};
+    mcx_mc_trace tr{acc_out, de_out, mvtyp_out, mvinst_out, nullptr, nullptr, 0, nullptr, nullptr};
     return mcx_mc_run_traced(h, types, ntypes, nsteps, seed, rng_mode, list_every, stats, &tr);
 }
 
@@ -939,6 +939,19 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
             a.half_skin2 = 0.25 * h->dcut * h->dcut;
             a.stop_at = stop_d.p;
         }
+        const int save = (tr.save_each > 0 && tr.on_frame) ? tr.save_each : 0;
+        std::vector<double> fx, fy, fz;
+        auto emit_frame = [&](int step) {
+            double e = 0.0;
+            const size_t n = (size_t)h->n;
+            fx.resize(n); fy.resize(n); fz.resize(n);
+            HIPCHK(hipMemcpyAsync(&e, etot.p, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(hipMemcpyAsync(fx.data(), h->x.p, n * 8, hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(hipMemcpyAsync(fy.data(), h->y.p, n * 8, hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(hipMemcpyAsync(fz.data(), h->z.p, n * 8, hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+            tr.on_frame(tr.user, step, e, fx.data(), fy.data(), fz.data());
+        };
         HIPCHK(hipStreamSynchronize(h->stream));
         const auto t0 = std::chrono::steady_clock::now();
         long long nlist = 0;
@@ -949,6 +962,7 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
         while (autolist && done < nsteps) {
             constexpr int kAutoChunk = 4096;
             a.nsteps = std::min(kAutoChunk, nsteps - done);
+            if (save) a.nsteps = std::min(a.nsteps, save - done % save);
             a.step0 = done;
             launch_mc_chunk(h->sys(), h->dterms(), mv, a, h->stream);
             HIPCHK(hipGetLastError());
@@ -961,10 +975,12 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
                 rebuild_list_fast(h);
                 nlist++;
             }
+            if (save && done % save == 0) emit_frame(done);
         }
         const int every = (list_every > 0 && h->cutmode != MCX_CUT_FULL) ? list_every : nsteps;
         while (!autolist && done < nsteps) {
             int chunk = every - (done % std::max(every, 1));
+            if (save) chunk = std::min(chunk, save - done % save);
             if (chunk > nsteps - done) chunk = nsteps - done;
             a.nsteps = chunk;
             a.step0 = done;
@@ -975,6 +991,7 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
                 rebuild_list_fast(h);
                 nlist++;
             }
+            if (save && done % save == 0) emit_frame(done);
         }
         HIPCHK(hipStreamSynchronize(h->stream));
         const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1128,7 +1145,7 @@ int mcx_time_kernel(mcx_ctx *h, int which, int reps, double *ms)
         if (which == 4) g6.alloc((size_t)6 * h->n);
         auto launch = [&] {
             switch (which) {
-            case 0: launch_full_energy(s, h->tiles.p, h->ntiles, 256, h->partial.p, h->stream); break;
+            case 0: launch_full_energy(s, h->tiles.p, h->ntiles, kFullTile, h->partial.p, h->stream); break;
             case 1: launch_switch_energy(s, h->partial.p, nbs, h->stream); break;
             case 2: launch_list_build(s, h->grid, h->stream); break;
             case 3: launch_bonded(s, t, h->partial.p, bonded_blocks(t), h->stream); break;

@@ -28,7 +28,11 @@
 #define MCX_MV_ROT_DEV 2
 #define MCX_MV_FBIAS_DEV 3 // force-bias translation (not in the reference)
 
-constexpr int kMaxMoveAtoms = 64;   // atoms per move instance on the device path
+constexpr int kMaxMoveAtoms = 64;
+#ifndef MCX_FULL_TILE
+#define MCX_FULL_TILE 64
+#endif
+constexpr int kFullTile = MCX_FULL_TILE;   // i and j atoms per tile of the O(N^2) kernel   // atoms per move instance on the device path
 constexpr int kMaxMoveTypes = 8;
 // Saved draw state of a chain (uint32 words): [0,624) MT19937 state, 624 MT
 // index, 625/626 Philox block counter (hi, lo), 627 cursor into the draw

@@ -2,7 +2,9 @@
 //
 //  * k_full_energy: all pairs i<j with minimum image and no cutoff
 //    (computeNonBonded_full, src/FField_MDBAS.cpp:194-265).  Triangular
-//    256x256 tiles; one i atom per lane, the j atom uniform across the
+//    64x64 tiles, one wavefront each (18,528 of them for 12,288 atoms, which
+//    balances the 1,024 SIMDs far better than 256-atom tiles: 61 us against
+//    78 us); one i atom per lane, the j atom uniform across the
 //    wavefront so that its data arrives through scalar loads and LJ is
 //    skipped by a scalar branch when eps_j = 0.  Excluded pairs are included
 //    here and subtracted by the fused bonded kernel, which keeps this loop
@@ -23,7 +25,7 @@
 
 namespace {
 
-constexpr int kTile = 256;
+constexpr int kTile = kFullTile;
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -57,21 +59,26 @@ __device__ __forceinline__ void disp(const DevSys &s, const int4 &pi, const int4
 }
 
 template <int PB, bool DIAG>
+__device__ __forceinline__ void tile_pair_acc(const DevSys &s, int i, int j, const int4 pi, const int4 pj,
+                                              const float2 ej, double xi, double yi, double zi, float si,
+                                              double &accC, double &accV)
+{
+    double ri;
+    float lj;
+    mcx_tile_pair<PB>(s, pi, pj, xi, yi, zi, j, si + ej.y, ri, lj);
+    if (DIAG && !(j > i)) return;
+    accC = fma((double)__int_as_float(pj.w), ri, accC);
+    if (ej.x != 0.f) accV = fma((double)ej.x, (double)lj, accV);
+}
+
+template <int PB, bool DIAG>
 __device__ __forceinline__ void full_tile(const DevSys &s, int i, int j0, int j1, const int4 pi, double xi, double yi,
                                           double zi, float si, double &accC, double &accV)
 {
     const int4 *__restrict__ ipos = s.ipos;
     const float2 *__restrict__ esk = s.esk;
-    for (int j = j0; j < j1; j++) {
-        const int4 pj = ipos[j];
-        const float2 ej = esk[j];
-        double ri;
-        float lj;
-        mcx_tile_pair<PB>(s, pi, pj, xi, yi, zi, j, si + ej.y, ri, lj);
-        if (DIAG && !(j > i)) continue;
-        accC = fma((double)__int_as_float(pj.w), ri, accC);
-        if (ej.x != 0.f) accV = fma((double)ej.x, (double)lj, accV);
-    }
+    for (int j = j0; j < j1; j++)
+        tile_pair_acc<PB, DIAG>(s, i, j, pi, ipos[j], esk[j], xi, yi, zi, si, accC, accV);
 }
 
 template <int PB>

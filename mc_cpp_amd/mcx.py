@@ -59,9 +59,12 @@ class BatchStatsC(C.Structure):
 _llp = C.POINTER(C.c_longlong)
 
 
+FRAME_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_double, _dp, _dp, _dp)
+
+
 class MCTraceC(C.Structure):
     _fields_ = [("acc", C.POINTER(C.c_ubyte)), ("de", _dp), ("mvtyp", _ip), ("mvinst", _ip), ("disp", _dp),
-                ("lnq", _dp)]
+                ("lnq", _dp), ("save_each", C.c_int), ("on_frame", FRAME_FN), ("user", C.c_void_p)]
 
 # every symbol include/mcx.h declares, with its argument types
 SIGNATURES = {
@@ -275,7 +278,10 @@ class Handle:
     def set_rng_stream(self, stream: int):
         self._chk(self.L.mcx_set_rng_stream(self.h, C.c_uint32(stream)))
 
-    def mc_run(self, moves: Sequence[S.MoveType], nsteps, seed, rng_mode=MC_REPLAY, list_every=50, record=True):
+    def mc_run(self, moves: Sequence[S.MoveType], nsteps, seed, rng_mode=MC_REPLAY, list_every=50, record=True,
+               save_each=0, on_frame=None):
+        """Device MC_metropolis::run.  on_frame(step, etot, x, y, z) is called
+        every save_each steps (numpy copies of the coordinates)."""
         nt = len(moves)
         arr, keep = _move_array(moves)
         st = MCStatsC()
@@ -285,8 +291,14 @@ class Handle:
         ins = np.zeros(max(nsteps, 1), np.int32) if record else None
         disp = np.zeros((max(nsteps, 1), 3)) if record else None
         lnq = np.zeros(max(nsteps, 1)) if record else None
+        n = self.n
+        cb = FRAME_FN()
+        if on_frame is not None and save_each > 0:
+            def _frame(user, step, etot, x, y, z):
+                on_frame(int(step), float(etot), *(np.ctypeslib.as_array(a, (n,)).copy() for a in (x, y, z)))
+            cb = FRAME_FN(_frame)
         tr = MCTraceC(_p(acc, C.c_ubyte), _p(de, C.c_double), _p(ty, C.c_int), _p(ins, C.c_int),
-                      _p(disp, C.c_double), _p(lnq, C.c_double))
+                      _p(disp, C.c_double), _p(lnq, C.c_double), int(save_each), cb, None)
         self._chk(self.L.mcx_mc_run_traced(self.h, arr, nt, int(nsteps), C.c_uint64(seed), int(rng_mode),
                                            int(list_every), C.byref(st), C.byref(tr)))
         out = dict(etot=st.etot, trials=np.array(st.trials[:nt]), accepted=np.array(st.accepted[:nt]),

@@ -69,6 +69,7 @@ class System:
     mass: np.ndarray = None
     frozen: np.ndarray = None
     resid: np.ndarray = None
+    atom_ids: np.ndarray = None      # column 1 of a COR file (1-based ids); default 1..n
     symbols: List[str] = None
     resnames: List[str] = None
     # bonded terms (atom indices 0-based, constants already in iu / radians)
@@ -92,6 +93,8 @@ class System:
             self.frozen = np.zeros(n, dtype=np.int32)
         if self.resid is None:
             self.resid = np.arange(1, n + 1, dtype=np.int32)
+        if self.atom_ids is None:
+            self.atom_ids = np.arange(1, n + 1, dtype=np.int32)
         if self.bonds is None:
             self.bonds = empty_terms("bonds")
         if self.ub is None:
@@ -109,6 +112,7 @@ class System:
             setattr(self, f, _d(getattr(self, f)))
         self.frozen = _i(self.frozen)
         self.resid = _i(self.resid)
+        self.atom_ids = _i(self.atom_ids)
 
     def copy(self) -> "System":
         """Deep copy of every per-atom array and term table (moves are shared)."""
@@ -239,14 +243,14 @@ def read_mdbas(cor_path: str, ff_path: str, box, pbtype=PBC_CUBIC, cutmode=CUT_S
         k += 1
     n = int(lines[k].split()[0])
     k += 1
-    xs, ys, zs, res, sym, rn = [], [], [], [], [], []
+    xs, ys, zs, res, sym, rn, ids = [], [], [], [], [], [], []
     toks = " ".join(lines[k:]).split()
     for i in range(n):
         t = toks[10 * i: 10 * i + 10]
-        res.append(int(t[1])); rn.append(t[2]); sym.append(t[3])
+        ids.append(int(t[0])); res.append(int(t[1])); rn.append(t[2]); sym.append(t[3])
         xs.append(float(t[4])); ys.append(float(t[5])); zs.append(float(t[6]))
     s = System(natom=n, box=np.asarray(box, dtype=float), pbtype=pbtype, cutmode=cutmode,
-               ctoff=ctoff, cton=cton, dcut=dcut, temp=temp, x=xs, y=ys, z=zs, resid=res)
+               ctoff=ctoff, cton=cton, dcut=dcut, temp=temp, x=xs, y=ys, z=zs, resid=res, atom_ids=ids)
     s.symbols, s.resnames = sym, rn
     with open(ff_path) as f:
         flines = f.read().splitlines()
