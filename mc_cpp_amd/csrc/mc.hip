@@ -36,6 +36,7 @@ constexpr int kMCW = kMCT / 64;
 struct MoveLDS {
     int first, last, nm, total, acc, type, inst, pivot;
     int scan;                    // this trial leaves the list's validity: every atom is a candidate
+    int rigid;                   // the set moves rigidly: terms among its own atoms are unchanged
     int stop;                    // an accepted move left it: end the chunk for a rebuild
     double r[3], R[3][3], piv[3];
     double tx[kMaxMoveAtoms], ty[kMaxMoveAtoms], tz[kMaxMoveAtoms];
@@ -102,7 +103,7 @@ __device__ __forceinline__ void pair_delta(const DevSys &s, const MoveLDS &m, in
 // drift): pairs of a moved atom with an atom outside the set, over the same
 // list rows as move_delta; pairs inside the set cancel.  Result in m.fold
 // after the final barrier.
-__device__ void force_old(const DevSys &s, MoveLDS &m)
+__device__ __forceinline__ void force_old(const DevSys &s, MoveLDS &m)
 {
     const int tid = threadIdx.x;
     double f[3] = {0.0, 0.0, 0.0};
@@ -143,18 +144,27 @@ __device__ void force_old(const DevSys &s, MoveLDS &m)
 }
 
 
+// Does the moved set hold every atom of term record id?  (Unused slots are
+// negative.)
+__device__ __forceinline__ bool set_holds(const MoveLDS &m, const int4 id)
+{
+    return id.x >= m.first && id.x <= m.last && id.y >= m.first && id.y <= m.last &&
+           (id.z < 0 || (id.z >= m.first && id.z <= m.last)) && (id.w < 0 || (id.w >= m.first && id.w <= m.last));
+}
+
 // dE of the move described in m (trial coordinates, displacements and the
 // cached moved-atom data set by make_trial).  Wavefront 0 evaluates the
-// bonded and 1-4 terms while the other wavefronts evaluate the nonbonded
-// pairs, each thread issuing the loads of kPairU list entries together.
+// bonded and 1-4 terms, less those among the atoms of a rigidly moved set,
+// while the other wavefronts evaluate the nonbonded pairs, each thread
+// issuing the loads of kPairU list entries together.
 // Returns the total in m.de (valid after the final barrier).
-__device__ void move_delta(const DevSys &s, const DevTerms &t, const DevMoves &mv, MoveLDS &m)
+__device__ __forceinline__ void move_delta(const DevSys &s, const DevTerms &t, const DevMoves &mv, MoveLDS &m)
 {
     const int tid = threadIdx.x;
     double dec = 0.0, dev = 0.0, deb = 0.0;
     double fnew[3] = {0.0, 0.0, 0.0};
     const bool fb = m.type == MCX_MV_FBIAS_DEV;
-    if (tid >= 64) {
+    if (__builtin_amdgcn_readfirstlane(tid) >= 64) {   // wave-uniform role
         constexpr int NP = kMCT - 64;
         const int pt = tid - 64;
         const int total = m.total, nm = m.nm;
@@ -207,6 +217,7 @@ __device__ void move_delta(const DevSys &s, const DevTerms &t, const DevMoves &m
             const int r = e - m.ra0;
             const bool lds = r < kMaxRec;
             const int4 id = lds ? m.rid[r] : mv.rec_ids[e];
+            if (m.rigid && set_holds(m, id)) continue;
             const int kk = lds ? m.rkk[r] : mv.rec_kk[e];
             const int ty = lds ? m.rty[r] : mv.rec_type[e];
             const double4 par = lds ? m.rpar[r] : mv.rec_par[e];
@@ -232,7 +243,7 @@ __device__ void move_delta(const DevSys &s, const DevTerms &t, const DevMoves &m
             const int ids[4] = {id.x, id.y, id.z, id.w};
             for (int q = 0; q < 4; q++)
                 if (ids[q] >= m.first && ids[q] <= m.last && ids[q] < lowest) lowest = ids[q];
-            if (lowest != owner) continue;
+            if (lowest != owner || (m.rigid && set_holds(m, id))) continue;
             double e0, c0, v0, e1, c1, v1;
             mcx_term_energy_rec(s, kk >> 28, id, mv.adj_type[e], mv.adj_par[e], fo, e0, c0, v0);
             mcx_term_energy_rec(s, kk >> 28, id, mv.adj_type[e], mv.adj_par[e], fn, e1, c1, v1);
@@ -266,7 +277,7 @@ __device__ void move_delta(const DevSys &s, const DevTerms &t, const DevMoves &m
 
 // Phase B: every datum of the moved atoms, the pivot and the move's term
 // records in one round of parallel loads.
-__device__ void load_moved(const DevSys &s, const DevMoves &mv, MoveLDS &m)
+__device__ __forceinline__ void load_moved(const DevSys &s, const DevMoves &mv, MoveLDS &m)
 {
     const int a = threadIdx.x;
     if (a < m.nm) {
@@ -278,6 +289,7 @@ __device__ void load_moved(const DevSys &s, const DevMoves &mv, MoveLDS &m)
         m.ms[a] = mv.mass ? mv.mass[i] : 1.0;
     } else if (a == kMaxMoveAtoms) {
         m.scan = 0;
+        m.rigid = 1;
         const int pv = m.pivot;
         if (pv >= 0 && (pv < m.first || pv > m.last)) {
             m.xpiv[0] = s.x[pv]; m.xpiv[1] = s.y[pv]; m.xpiv[2] = s.z[pv];
@@ -306,7 +318,7 @@ __device__ __forceinline__ bool beyond_half_skin(const DevSys &s, int4 p, int4 r
 
 // Phase C: trial coordinates (MOVE_TRN::tr_a_b / MOVE_ROT::apply_rotation,
 // include/Move.hpp:42-50,120-143, each atom re-wrapped) and displacements.
-__device__ void make_trial(const DevSys &s, MoveLDS &m, const int4 *ipos_ref, double half_skin2)
+__device__ __forceinline__ void make_trial(const DevSys &s, MoveLDS &m, const int4 *ipos_ref, double half_skin2)
 {
     const int a = threadIdx.x;
     if (a < m.nm) {
@@ -330,6 +342,15 @@ __device__ void make_trial(const DevSys &s, MoveLDS &m, const int4 *ipos_ref, do
                 piv[0] = c0 / mt_; piv[1] = c1 / mt_; piv[2] = c2 / mt_;
             }
             const double px = x - piv[0], py = y - piv[1], pz = z - piv[2];
+            // A rotation whose atoms all lie within a quarter box of the
+            // pivot moves the set rigidly, like a translation: distances,
+            // angles and torsions inside it change by rounding only
+            // (~1e-15 relative) and the terms among its own atoms are
+            // skipped.  The pivot is applied to wrapped coordinates
+            // (include/Move.hpp:120-143), so a set the box edge splits is
+            // not rigid and keeps them.
+            const double lq = 0.25 * fmin(s.L[0], fmin(s.L[1], s.L[2]));
+            if (s.pbtype != 0 && px * px + py * py + pz * pz >= lq * lq) atomicAnd(&m.rigid, 0);
             ux = piv[0] + m.R[0][0] * px + m.R[1][0] * py + m.R[2][0] * pz;
             uy = piv[1] + m.R[0][1] * px + m.R[1][1] * py + m.R[2][1] * pz;
             uz = piv[2] + m.R[0][2] * px + m.R[1][2] * py + m.R[2][2] * pz;
@@ -345,7 +366,7 @@ __device__ void make_trial(const DevSys &s, MoveLDS &m, const int4 *ipos_ref, do
 
 // Prefix of the moved atoms' list-row lengths (the flattened pair index of
 // move_delta and force_old).
-__device__ void row_prefix(const DevSys &s, MoveLDS &m)
+__device__ __forceinline__ void row_prefix(const DevSys &s, MoveLDS &m)
 {
     if (threadIdx.x == 64) {
         if (m.scan)
@@ -368,7 +389,7 @@ __global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s, DevTerms t, DevMove
     __shared__ double etot;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const bool w0 = tid < 64;
+    const bool w0 = __builtin_amdgcn_readfirstlane(tid) < 64;   // wave-uniform
     WaveDraws dr;
     if (w0) {
         dr.replay = arg.rng_mode == 1;
@@ -510,6 +531,7 @@ __global__ __launch_bounds__(kMCT) void k_delta(DevSys s, DevTerms t, DevMoves m
         for (int b = 0; b < nm; b++) { m.pref[b] = acc; acc += m.mc[b]; }
         m.pref[nm] = acc;
         m.total = acc;
+        m.rigid = 0;      // arbitrary trial coordinates
     }
     __syncthreads();
     move_delta(s, t, mv, m);

@@ -47,7 +47,7 @@ __device__ __forceinline__ void wave_sync()
 // mt[i+1] and on mt[i+397] (old for i < 227, new for i >= 227), so the
 // ranges [0,227), [227,454), [454,623) and {623} are each free of
 // intra-phase dependencies; every phase reads all its inputs before writing.
-__device__ void mt_twist_wave(uint32_t *mt, int lane)
+__device__ __forceinline__ void mt_twist_wave(uint32_t *mt, int lane)
 {
     const int lo[4] = {0, 227, 454, 623};
     const int hi[4] = {227, 454, 623, 624};
@@ -80,7 +80,7 @@ __device__ __forceinline__ double canonical2(uint32_t lo, uint32_t hi)
     return r;
 }
 
-__device__ uint4 philox(uint4 c, uint2 k)
+__device__ __forceinline__ uint4 philox(uint4 c, uint2 k)
 {
     for (int r = 0; r < 10; r++) {
         const uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
@@ -109,14 +109,14 @@ struct WaveDraws {
     uint32_t chain;    // Philox stream
     uint32_t *mt;      // LDS state
 
-    __device__ void load(const uint32_t *st, int lane)
+    __device__ __forceinline__ void load(const uint32_t *st, int lane)
     {
         mt_idx = (int)st[624];
         ctr = ((uint64_t)st[625] << 32) | st[626];
         cur = (int)st[627];
         v = lane < 32 ? __hiloint2double((int)st[628 + 2 * lane + 1], (int)st[628 + 2 * lane]) : 0.0;
     }
-    __device__ void save(uint32_t *st, int lane) const
+    __device__ __forceinline__ void save(uint32_t *st, int lane) const
     {
         if (lane == 0) {
             st[624] = (uint32_t)mt_idx;
@@ -129,7 +129,7 @@ struct WaveDraws {
             st[628 + 2 * lane + 1] = (uint32_t)__double2hiint(v);
         }
     }
-    __device__ void refill(int lane)
+    __device__ __forceinline__ void refill(int lane)
     {
         if (replay) {
             const int p = mt_idx + lane;
@@ -159,7 +159,7 @@ struct WaveDraws {
         }
         cur = 0;
     }
-    __device__ double next(int lane)
+    __device__ __forceinline__ double next(int lane)
     {
         if (cur == 32) refill(lane);
         const double u = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), cur),
@@ -184,7 +184,7 @@ __device__ __forceinline__ double wmax(double v)
 }
 
 // find_rot_mat, include/Move.hpp:77-118
-__device__ void rot_mat(double mat[3][3], double vec[3], double angle)
+__device__ __forceinline__ void rot_mat(double mat[3][3], double vec[3], double angle)
 {
     const double angleRad = angle * MCX_PI / 180.0;
     const double rn = 1.0 / sqrt(vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2]);
@@ -216,7 +216,7 @@ __device__ __forceinline__ double wrap1(const DevSys &s, double v, int k)
 // (Box-Muller; left in m.r for the drift to be added).  Lane 0 publishes the
 // move in m.
 template <class M>
-__device__ void pick_move(const DevMoves &mv, WaveDraws &dr, const double *dmax, M &m, int lane)
+__device__ __forceinline__ void pick_move(const DevMoves &mv, WaveDraws &dr, const double *dmax, M &m, int lane)
 {
     const int tp = (int)((double)mv.ntypes * dr.next(lane));
     const int in = (int)((double)mv.ninst[tp] * dr.next(lane));

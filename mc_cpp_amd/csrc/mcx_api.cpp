@@ -371,6 +371,9 @@ void upload_mirror(mcx_ctx *h)
     const double sc = h->pbtype == MCX_PBC_CUBIC ? 4294967296.0 / h->box[0] : 1.0;
     for (int i = 0; i < n; i++) ek[i] = make_float2((float)h->eps[i], (float)(h->sig[i] * sc));
     h->esk.upload(ek.data(), n, h->stream);
+    std::vector<TileAtom> tq(n);
+    for (int i = 0; i < n; i++) tq[i] = TileAtom{(double)__builtin_bit_cast(float, m[i].w), ek[i].x, ek[i].y};
+    h->tq.upload(tq.data(), n, h->stream);
     HIPCHK(hipStreamSynchronize(h->stream));
 }
 

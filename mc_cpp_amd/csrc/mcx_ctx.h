@@ -70,6 +70,7 @@ struct mcx_ctx {
     DBuf<double> x, y, z, q64, eps64, sig64, eps1464, sig1464, mass64;
     DBuf<int4> ipos;
     DBuf<float2> es, esk, es14;
+    DBuf<TileAtom> tq;
     DBuf<double4> qes;
     std::vector<double> hx, hy, hz;   // host copy of the last uploaded coordinates
     DBuf<int> frozen_d, excl_off_d, excl_idx_d, xfull_off_d, xfull_idx_d;
@@ -115,7 +116,7 @@ struct mcx_ctx {
         s.cutmode = cutmode;
         s.x = x.p; s.y = y.p; s.z = z.p;
         s.ipos = ipos.p;
-        s.es = es.p; s.esk = esk.p; s.es14 = es14.p;
+        s.es = es.p; s.esk = esk.p; s.tq = tq.p; s.es14 = es14.p;
         for (int k = 0; k < 3; k++) s.unit[k] = pbtype == MCX_PBC_NONE ? 1.0 : box[k] / 4294967296.0;
         s.iunit = pbtype == MCX_PBC_CUBIC ? 4294967296.0 / box[0] : 1.0;
         s.qes = qes.p;
@@ -152,7 +153,7 @@ struct mcx_ctx {
         DBuf<double> *d[] = {&x, &y, &z, &q64, &eps64, &sig64, &eps1464, &sig1464, &mass64, &partial, &result,
                              &newxyz, &scalar};
         for (auto *b : d) b->release();
-        ipos.release(); es.release(); esk.release(); es14.release(); qes.release();
+        ipos.release(); es.release(); esk.release(); tq.release(); es14.release(); qes.release();
         cpos.release();
         cfix.release();
         ipos_ref.release();

@@ -28,11 +28,11 @@
 #define MCX_MV_ROT_DEV 2
 #define MCX_MV_FBIAS_DEV 3 // force-bias translation (not in the reference)
 
-constexpr int kMaxMoveAtoms = 64;
+constexpr int kMaxMoveAtoms = 64;           // atoms per move instance on the device path
 #ifndef MCX_FULL_TILE
 #define MCX_FULL_TILE 64
 #endif
-constexpr int kFullTile = MCX_FULL_TILE;   // i and j atoms per tile of the O(N^2) kernel   // atoms per move instance on the device path
+constexpr int kFullTile = MCX_FULL_TILE;   // i and j atoms per tile of the O(N^2) kernel
 constexpr int kMaxMoveTypes = 8;
 // Saved draw state of a chain (uint32 words): [0,624) MT19937 state, 624 MT
 // index, 625/626 Philox block counter (hi, lo), 627 cursor into the draw
@@ -40,6 +40,14 @@ constexpr int kMaxMoveTypes = 8;
 constexpr int kRngWords = 692;
 
 // Everything a kernel needs about one system, passed by value.
+// Per-atom parameters of the all-pair kernel's j atom, read as one 16-byte
+// scalar load: q (the FP32 value of ipos.w, widened to FP64 so that it is
+// an FP64 operand as it stands) and eps, sig (esk).
+struct alignas(16) TileAtom {
+    double q;
+    float eps, sig;
+};
+
 struct DevSys {
     int n;
     int pbtype;
@@ -57,6 +65,7 @@ struct DevSys {
     double iunit;        // 2^32 / L for cubic boxes (kernel length unit), else 1
     float2 *es;          // (eps, sig) FP32, sig in A
     float2 *esk;         // (eps, sig) FP32, sig in kernel length units
+    TileAtom *tq;        // q of ipos.w widened to FP64, and esk: the all-pair kernel's j data
     float2 *es14;        // (eps14, sig14)
     const double *q64, *eps64, *sig64, *eps1464, *sig1464;
     const double4 *qes;  // (charge, eps, sig in A, 0) FP64 per atom: the per-move dE path

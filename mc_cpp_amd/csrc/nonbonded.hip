@@ -58,27 +58,64 @@ __device__ __forceinline__ void disp(const DevSys &s, const int4 &pi, const int4
     mcx_disp_k<PB>(s, pi, pj, xi, yi, zi, j, dx, dy, dz);
 }
 
-template <int PB, bool DIAG>
-__device__ __forceinline__ void tile_pair_acc(const DevSys &s, int i, int j, const int4 pi, const int4 pj,
-                                              const float2 ej, double xi, double yi, double zi, float si,
-                                              double &accC, double &accV)
+// Is the wave-uniform value v non-zero in any bit?  A scalar compare: the
+// compiler would otherwise test a uniform float with a vector instruction.
+__device__ __forceinline__ bool scalar_nonzero(float v)
 {
-    double ri;
-    float lj;
-    mcx_tile_pair<PB>(s, pi, pj, xi, yi, zi, j, si + ej.y, ri, lj);
-    if (DIAG && !(j > i)) return;
-    accC = fma((double)__int_as_float(pj.w), ri, accC);
-    if (ej.x != 0.f) accV = fma((double)ej.x, (double)lj, accV);
+    int r;
+    asm("s_cmp_lg_u32 %1, 0\n\ts_cselect_b32 %0, 1, 0" : "=s"(r) : "s"(v) : "scc");
+    return r != 0;
 }
 
+// One j atom of a tile against this lane's i atom.  j is uniform across the
+// wavefront, so its position and parameters arrive as scalar operands, and
+// the test for eps_j == 0 (the water hydrogens) is a scalar bit test: the
+// Lennard-Jones part is a scalar branch.
+template <int PB, bool DIAG>
+__device__ __forceinline__ void tile_pair_acc(const DevSys &s, int i, int j, const int4 pi, const int4 pj,
+                                              const TileAtom &qe, double xi, double yi, double zi, float si,
+                                              double &accC, double &accV)
+{
+    double r0;
+    const double ri = mcx_tile_ri<PB>(s, pi, pj, xi, yi, zi, j, r0);
+    if (DIAG && !(j > i)) return;
+    accC = fma(qe.q, ri, accC);
+    if (scalar_nonzero(qe.eps)) accV = fma((double)qe.eps, (double)mcx_tile_lj(si + qe.sig, r0), accV);
+}
+
+#ifndef MCX_TILE_ILP
+#define MCX_TILE_ILP 2
+#endif
+
+// The j atoms of a tile in order.  Off the diagonal, two j atoms go
+// together: their distance chains and 1/r estimates are independent, then
+// each Lennard-Jones branch, then both Newton steps, with separate
+// accumulators so that the two chains overlap.
 template <int PB, bool DIAG>
 __device__ __forceinline__ void full_tile(const DevSys &s, int i, int j0, int j1, const int4 pi, double xi, double yi,
                                           double zi, float si, double &accC, double &accV)
 {
     const int4 *__restrict__ ipos = s.ipos;
-    const float2 *__restrict__ esk = s.esk;
-    for (int j = j0; j < j1; j++)
-        tile_pair_acc<PB, DIAG>(s, i, j, pi, ipos[j], esk[j], xi, yi, zi, si, accC, accV);
+    const TileAtom *__restrict__ tq = s.tq;
+    int j = j0;
+    if (!DIAG && MCX_TILE_ILP == 2) {
+        double accC2 = 0.0, accV2 = 0.0;
+        for (; j + 1 < j1; j += 2) {
+            const int4 pa = ipos[j], pb = ipos[j + 1];
+            const TileAtom qa = tq[j], qb = tq[j + 1];
+            const double r2a = mcx_tile_r2<PB>(s, pi, pa, xi, yi, zi, j);
+            const double r2b = mcx_tile_r2<PB>(s, pi, pb, xi, yi, zi, j + 1);
+            const double r0a = __builtin_amdgcn_rsq(r2a), r0b = __builtin_amdgcn_rsq(r2b);
+            if (scalar_nonzero(qa.eps)) accV = fma((double)qa.eps, (double)mcx_tile_lj(si + qa.sig, r0a), accV);
+            if (scalar_nonzero(qb.eps)) accV2 = fma((double)qb.eps, (double)mcx_tile_lj(si + qb.sig, r0b), accV2);
+            accC = fma(qa.q, mcx_tile_newton(r2a, r0a), accC);
+            accC2 = fma(qb.q, mcx_tile_newton(r2b, r0b), accC2);
+        }
+        accC += accC2;
+        accV += accV2;
+    }
+    for (; j < j1; j++)
+        tile_pair_acc<PB, DIAG>(s, i, j, pi, ipos[j], tq[j], xi, yi, zi, si, accC, accV);
 }
 
 template <int PB>

@@ -110,22 +110,52 @@ __device__ __forceinline__ void mcx_disp_k(const DevSys &s, const int4 &pi, cons
     }
 }
 
-// The two per-pair values of the all-pair kernel: 1/r (FP64, kernel units)
-// and t^6 - t^3 (FP32).  The excluded-pair correction calls this too, so a
-// pair added by the tile kernel is removed with identical values.
+// The per-pair values of the all-pair kernel, in two parts so that the
+// Lennard-Jones part is skipped for atoms without it: 1/r in FP64 (kernel
+// length units) from the FP64 hardware estimate (relative error <= 2^-23)
+// and one Newton step (~2e-14), returning the estimate r0 too; and
+// t^6 - t^3 in FP32 with t = (sij * r0)^2.  The excluded-pair correction
+// calls the same two, so a pair added by the tile kernel is removed with
+// identical values.
+template <int PB>
+__device__ __forceinline__ double mcx_tile_r2(const DevSys &s, const int4 &pi, const int4 &pj, double xi, double yi,
+                                              double zi, int j)
+{
+    double dx, dy, dz;
+    mcx_disp_k<PB>(s, pi, pj, xi, yi, zi, j, dx, dy, dz);
+    return fma(dz, dz, fma(dy, dy, dx * dx));
+}
+
+__device__ __forceinline__ double mcx_tile_newton(double r2, double r0)
+{
+    const double e = fma(-r2 * r0, r0, 1.0);
+    return fma(0.5 * r0, e, r0);
+}
+
+template <int PB>
+__device__ __forceinline__ double mcx_tile_ri(const DevSys &s, const int4 &pi, const int4 &pj, double xi, double yi,
+                                              double zi, int j, double &r0)
+{
+    const double r2 = mcx_tile_r2<PB>(s, pi, pj, xi, yi, zi, j);
+    r0 = __builtin_amdgcn_rsq(r2);
+    return mcx_tile_newton(r2, r0);
+}
+
+__device__ __forceinline__ float mcx_tile_lj(float sij, double r0)
+{
+    const float u = sij * (float)r0;
+    const float t = u * u;
+    const float t3 = t * t * t;
+    return fmaf(t3, t3, -t3);
+}
+
 template <int PB>
 __device__ __forceinline__ void mcx_tile_pair(const DevSys &s, const int4 &pi, const int4 &pj, double xi, double yi,
                                               double zi, int j, float sij, double &ri, float &lj)
 {
-    double dx, dy, dz;
-    mcx_disp_k<PB>(s, pi, pj, xi, yi, zi, j, dx, dy, dz);
-    const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-    float ri32;
-    ri = mcx_rsqrt(r2, ri32);
-    const float u = sij * ri32;
-    const float t = u * u;
-    const float t3 = t * t * t;
-    lj = fmaf(t3, t3, -t3);
+    double r0;
+    ri = mcx_tile_ri<PB>(s, pi, pj, xi, yi, zi, j, r0);
+    lj = mcx_tile_lj(sij, r0);
 }
 
 __device__ __forceinline__ void mcx_mimg64(const DevSys &s, double d[3])
