@@ -153,66 +153,72 @@ __device__ __forceinline__ bool set_holds(const MoveLDS &m, const int4 id)
 }
 
 // dE of the move described in m (trial coordinates, displacements and the
-// cached moved-atom data set by make_trial).  Wavefront 0 evaluates the
-// bonded and 1-4 terms, less those among the atoms of a rigidly moved set,
-// while the other wavefronts evaluate the nonbonded pairs, each thread
-// issuing the loads of kPairU list entries together.
-// Returns the total in m.de (valid after the final barrier).
-__device__ __forceinline__ void move_delta(const DevSys &s, const DevTerms &t, const DevMoves &mv, MoveLDS &m)
+// cached moved-atom data set by make_trial), in three parts: the nonbonded
+// pairs of the moved atoms' list rows (threads 64.., each issuing the loads
+// of kPairU list entries together), the bonded and 1-4 terms (wavefront 0,
+// less those among the atoms of a rigidly moved set), and the workgroup
+// reduction into m.de.
+
+// Nonbonded part (threads 64..): raw Coulomb and LJ sums, and with `fb` the
+// net force on the set at its new position.
+__device__ __forceinline__ void delta_pairs(const DevSys &s, const MoveLDS &m, bool fb, double &dec, double &dev,
+                                            double (&fnew)[3])
 {
-    const int tid = threadIdx.x;
-    double dec = 0.0, dev = 0.0, deb = 0.0;
-    double fnew[3] = {0.0, 0.0, 0.0};
-    const bool fb = m.type == MCX_MV_FBIAS_DEV;
-    if (__builtin_amdgcn_readfirstlane(tid) >= 64) {   // wave-uniform role
-        constexpr int NP = kMCT - 64;
-        const int pt = tid - 64;
-        const int total = m.total, nm = m.nm;
-        const bool all = s.cutmode == 0 || m.scan;
-        for (int base = 0; base < total; base += kPairU * NP) {
-            int av[kPairU], jv[kPairU];
+    constexpr int NP = kMCT - 64;
+    const int pt = threadIdx.x - 64;
+    const int total = m.total, nm = m.nm;
+    const bool all = s.cutmode == 0 || m.scan;
+    for (int base = 0; base < total; base += kPairU * NP) {
+        int av[kPairU], jv[kPairU];
 #pragma unroll
-            for (int u = 0; u < kPairU; u++) {
-                const int e = base + u * NP + pt;
-                av[u] = -1;
-                jv[u] = -1;
-                if (e < total) {
-                    int lo = 0, hi = nm - 1;       // last a with pref[a] <= e
-                    while (lo < hi) {
-                        const int mid = (lo + hi + 1) >> 1;
-                        if (m.pref[mid] <= e) lo = mid; else hi = mid - 1;
-                    }
-                    const int k = e - m.pref[lo];
-                    av[u] = lo;
-                    jv[u] = all ? k : s.nbr[(size_t)(m.first + lo) * s.cap + k];
+        for (int u = 0; u < kPairU; u++) {
+            const int e = base + u * NP + pt;
+            av[u] = -1;
+            jv[u] = -1;
+            if (e < total) {
+                int lo = 0, hi = nm - 1;       // last a with pref[a] <= e
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (m.pref[mid] <= e) lo = mid; else hi = mid - 1;
                 }
-            }
-            int4 pj[kPairU];
-            double4 pq[kPairU];
-#pragma unroll
-            for (int u = 0; u < kPairU; u++) {
-                if (jv[u] >= 0) { pj[u] = s.ipos[jv[u]]; pq[u] = s.qes[jv[u]]; }
-            }
-#pragma unroll
-            for (int u = 0; u < kPairU; u++) {
-                const int a = av[u], j = jv[u];
-                if (a < 0) continue;
-                const int i = m.first + a;
-                if (all && (j == i || excluded(s, i, j))) continue;
-                if (m.scan && s.frozen[i] && s.frozen[j]) continue;   // not a list pair
-                int b = -1;
-                if (j >= m.first && j <= m.last) {
-                    if (j < i) continue;     // pair of two moved atoms: counted once
-                    b = j - m.first;
-                }
-                pair_delta(s, m, i, a, j, b, pj[u], pq[u], dec, dev, fb, fnew);
+                const int k = e - m.pref[lo];
+                av[u] = lo;
+                jv[u] = all ? k : s.nbr[(size_t)(m.first + lo) * s.cap + k];
             }
         }
-    } else if (m.ra1 >= 0) {
-        // bonded and 1-4 terms of the move: self-contained records, staged in
-        // LDS by load_moved (or read from global memory past kMaxRec)
-        const CoordFetch fo{s.x, s.y, s.z, nullptr, nullptr, nullptr, 0, -1};
-        const CoordFetch fn{s.x, s.y, s.z, m.tx, m.ty, m.tz, m.first, m.last};
+        int4 pj[kPairU];
+        double4 pq[kPairU];
+#pragma unroll
+        for (int u = 0; u < kPairU; u++) {
+            if (jv[u] >= 0) { pj[u] = s.ipos[jv[u]]; pq[u] = s.qes[jv[u]]; }
+        }
+#pragma unroll
+        for (int u = 0; u < kPairU; u++) {
+            const int a = av[u], j = jv[u];
+            if (a < 0) continue;
+            const int i = m.first + a;
+            if (all && (j == i || excluded(s, i, j))) continue;
+            if (m.scan && s.frozen[i] && s.frozen[j]) continue;   // not a list pair
+            int b = -1;
+            if (j >= m.first && j <= m.last) {
+                if (j < i) continue;     // pair of two moved atoms: counted once
+                b = j - m.first;
+            }
+            pair_delta(s, m, i, a, j, b, pj[u], pq[u], dec, dev, fb, fnew);
+        }
+    }
+}
+
+// Bonded and 1-4 part (wavefront 0).
+__device__ __forceinline__ void delta_terms(const DevSys &s, const DevMoves &mv, const MoveLDS &m, double &dec,
+                                            double &dev, double &deb)
+{
+    const int tid = threadIdx.x;
+    const CoordFetch fo{s.x, s.y, s.z, nullptr, nullptr, nullptr, 0, -1};
+    const CoordFetch fn{s.x, s.y, s.z, m.tx, m.ty, m.tz, m.first, m.last};
+    if (m.ra1 >= 0) {
+        // self-contained records, staged in LDS by load_moved (or read from
+        // global memory past kMaxRec)
         for (int e = m.ra0 + tid; e < m.ra1; e += 64) {
             const int r = e - m.ra0;
             const bool lds = r < kMaxRec;
@@ -229,10 +235,8 @@ __device__ __forceinline__ void move_delta(const DevSys &s, const DevTerms &t, c
             dev += v1 - v0;
         }
     } else {
-        // bonded and 1-4 terms touching the moved atoms through the per-atom
-        // adjacency, each counted at its lowest moved atom
-        const CoordFetch fo{s.x, s.y, s.z, nullptr, nullptr, nullptr, 0, -1};
-        const CoordFetch fn{s.x, s.y, s.z, m.tx, m.ty, m.tz, m.first, m.last};
+        // terms touching the moved atoms through the per-atom adjacency,
+        // each counted at its lowest moved atom
         const int a0 = mv.adj_off[m.first], a1 = mv.adj_off[m.last + 1];
         for (int e = a0 + tid; e < a1; e += 64) {
             const int4 id = mv.adj_ids[e];
@@ -252,6 +256,14 @@ __device__ __forceinline__ void move_delta(const DevSys &s, const DevTerms &t, c
             dev += v1 - v0;
         }
     }
+}
+
+// Workgroup reduction of the parts: m.de (and m.fnew with fb) valid after
+// the final barrier.
+__device__ __forceinline__ void delta_finish(MoveLDS &m, bool fb, double dec, double dev, double deb,
+                                             double (&fnew)[3])
+{
+    const int tid = threadIdx.x;
     dec = wsum(dec);
     dev = wsum(dev);
     deb = wsum(deb);
@@ -275,6 +287,17 @@ __device__ __forceinline__ void move_delta(const DevSys &s, const DevTerms &t, c
     __syncthreads();
 }
 
+// All three parts (the one-shot dE kernel).
+__device__ __forceinline__ void move_delta(const DevSys &s, const DevTerms &t, const DevMoves &mv, MoveLDS &m)
+{
+    double dec = 0.0, dev = 0.0, deb = 0.0;
+    double fnew[3] = {0.0, 0.0, 0.0};
+    const bool fb = m.type == MCX_MV_FBIAS_DEV;
+    if (__builtin_amdgcn_readfirstlane(threadIdx.x) >= 64) delta_pairs(s, m, fb, dec, dev, fnew);
+    else delta_terms(s, mv, m, dec, dev, deb);
+    delta_finish(m, fb, dec, dev, deb, fnew);
+}
+
 // Phase B: every datum of the moved atoms, the pivot and the move's term
 // records in one round of parallel loads.
 __device__ __forceinline__ void load_moved(const DevSys &s, const DevMoves &mv, MoveLDS &m)
@@ -292,7 +315,12 @@ __device__ __forceinline__ void load_moved(const DevSys &s, const DevMoves &mv, 
         m.rigid = 1;
         const int pv = m.pivot;
         if (pv >= 0 && (pv < m.first || pv > m.last)) {
-            m.xpiv[0] = s.x[pv]; m.xpiv[1] = s.y[pv]; m.xpiv[2] = s.z[pv];
+            // wavefront 0 commits positions with vector stores; a plain load
+            // at this uniform address may be compiled as a scalar-cache load,
+            // which those stores do not update: read through atomic loads
+            m.xpiv[0] = __hip_atomic_load(s.x + pv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            m.xpiv[1] = __hip_atomic_load(s.y + pv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            m.xpiv[2] = __hip_atomic_load(s.z + pv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
     }
     const int r = a - 128;
@@ -408,37 +436,46 @@ __global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s, DevTerms t, DevMove
     if (tid == 0) m.stop = 0;
     __syncthreads();
 
+    // The step loop, in two copies with the same sequence of workgroup
+    // barriers: wavefront 0 runs the chain (pick, trial, bonded terms,
+    // Metropolis test, commit) and the other wavefronts the nonbonded pairs
+    // and the loads they are given.  Separate loops keep each role's values
+    // out of the other's register allocation.
     int step = 0;
-    for (; step < arg.nsteps; step++) {
-        const int st = arg.step0 + step + 1;
-        // ---- phase A (wavefront 0, lanes in step): pick and draw the move
-        // (src/MC_metropolis.cpp:85-116; src/MC.cpp:137-156)
-        if (w0 && !m.stop) pick_move(mv, dr, dmax, m, lane);
-        __syncthreads();
-        if (m.stop) break;
-        load_moved(s, mv, m);
-        row_prefix(s, m);
-        if (m.type == MCX_MV_FBIAS_DEV) {
-            // force-bias proposal: dx = beta A F + sqrt(2A) xi with F the net
-            // nonbonded force on the set in kcal/mol/A (iu/A / 418.4), A =
-            // dmax (A^2) and beta = 1/(R T) in mol/kcal
-            force_old(s, m);
-            if (tid == 0) {
-                const double A = dmax[m.acc], bA = arg.beta * A / MCX_KCALTOIU, sq = sqrt(2.0 * A);
-                for (int c = 0; c < 3; c++) {
-                    m.xi[c] = m.r[c];
-                    m.r[c] = bA * m.fold[c] + sq * m.xi[c];
-                }
-            }
+    if (w0) {
+        for (; step < arg.nsteps; step++) {
+            const int st = arg.step0 + step + 1;
+            // ---- phase A (lanes in step): pick and draw the move
+            // (src/MC_metropolis.cpp:85-116; src/MC.cpp:137-156)
+            if (!m.stop) pick_move(mv, dr, dmax, m, lane);
             __syncthreads();
-        }
-        make_trial(s, m, arg.ipos_ref, arg.half_skin2);
-        if (m.scan) row_prefix(s, m);     // every atom is a candidate of this trial
-        move_delta(s, t, mv, m);
-        // ---- phase E+F (wavefront 0): Metropolis test (apply_criterion,
-        // src/MC_metropolis.cpp:175-204), bookkeeping, commit of the moved
-        // atoms; the next phase-A barrier publishes the commit
-        if (w0) {
+            if (m.stop) break;
+            load_moved(s, mv, m);
+            __syncthreads();                  // row_prefix (thread 64)
+            const bool fb = m.type == MCX_MV_FBIAS_DEV;
+            if (fb) {
+                // force-bias proposal: dx = beta A F + sqrt(2A) xi with F the
+                // net nonbonded force on the set in kcal/mol/A (iu/A /
+                // 418.4), A = dmax (A^2) and beta = 1/(R T) in mol/kcal
+                force_old(s, m);
+                if (tid == 0) {
+                    const double A = dmax[m.acc], bA = arg.beta * A / MCX_KCALTOIU, sq = sqrt(2.0 * A);
+                    for (int c = 0; c < 3; c++) {
+                        m.xi[c] = m.r[c];
+                        m.r[c] = bA * m.fold[c] + sq * m.xi[c];
+                    }
+                }
+                __syncthreads();
+            }
+            make_trial(s, m, arg.ipos_ref, arg.half_skin2);
+            if (m.scan) __syncthreads();      // row_prefix over every atom (thread 64)
+            double dec = 0.0, dev = 0.0, deb = 0.0;
+            double fnew[3] = {0.0, 0.0, 0.0};
+            delta_terms(s, mv, m, dec, dev, deb);
+            delta_finish(m, fb, dec, dev, deb, fnew);
+            // ---- phase E+F: Metropolis test (apply_criterion,
+            // src/MC_metropolis.cpp:175-204), bookkeeping, commit of the
+            // moved atoms; the next phase-A barrier publishes the commit
             const int tp = m.acc;
             const double de = m.de;
             int acc;
@@ -487,6 +524,24 @@ __global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s, DevTerms t, DevMove
                 far = arg.ipos_ref && beyond_half_skin(s, np, arg.ipos_ref[i], arg.half_skin2);
             }
             if (__ballot(far) != 0ull && lane == 0) m.stop = 1;
+        }
+    } else {
+        for (; step < arg.nsteps; step++) {
+            __syncthreads();                  // phase A
+            if (m.stop) break;
+            load_moved(s, mv, m);
+            row_prefix(s, m);
+            const bool fb = m.type == MCX_MV_FBIAS_DEV;
+            if (fb) {
+                force_old(s, m);
+                __syncthreads();
+            }
+            __syncthreads();                  // make_trial (wavefront 0)
+            if (m.scan) row_prefix(s, m);     // every atom is a candidate of this trial
+            double dec = 0.0, dev = 0.0;
+            double fnew[3] = {0.0, 0.0, 0.0};
+            delta_pairs(s, m, fb, dec, dev, fnew);
+            delta_finish(m, fb, dec, dev, 0.0, fnew);
         }
     }
     if (arg.stop_at && tid == 0) {
