@@ -15,6 +15,7 @@
 // Philox4x32-10 stream), the test of apply_criterion (:175-204) and the
 // dmax adaptation (src/MC.cpp:165-173) all run in the same workgroup, so a
 // chunk of moves is one launch with no host round trip.
+#include "../../include/mcx.h"
 #include "mcmove.h"
 
 #include <cfloat>
@@ -407,8 +408,28 @@ __device__ __forceinline__ void row_prefix(const DevSys &s, MoveLDS &m)
     __syncthreads();
 }
 
-__global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s, DevTerms t, DevMoves mv, MCRunArgs arg)
+// The system with the box type and cutoff mode made compile-time
+// constants (PB, CUT >= 0): the branches on them fold away, and a cubic
+// box's three edges become one value.  PB or CUT < 0 keeps the field.
+template <int PB, int CUT>
+__device__ __forceinline__ DevSys specialise(const DevSys &s0)
 {
+    DevSys s = s0;
+    if (PB >= 0) s.pbtype = PB;
+    if (CUT >= 0) s.cutmode = CUT;
+    if (PB == MCX_PBC_CUBIC) {
+        for (int d = 1; d < 3; d++) {
+            s.L[d] = s0.L[0]; s.rL[d] = s0.rL[0]; s.unit[d] = s0.unit[0];
+            s.Lf[d] = s0.Lf[0]; s.rLf[d] = s0.rLf[0];
+        }
+    }
+    return s;
+}
+
+template <int PB, int CUT>
+__global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s0, DevTerms t, DevMoves mv, MCRunArgs arg)
+{
+    const DevSys s = specialise<PB, CUT>(s0);
     __shared__ MoveLDS m;
     __shared__ uint32_t mt[kMTN];
     __shared__ double dmax[kMaxMoveTypes];
@@ -603,5 +624,12 @@ void launch_delta_energy(const DevSys &s, const DevTerms &t, const DevMoves &mv,
 
 void launch_mc_chunk(const DevSys &s, const DevTerms &t, const DevMoves &mv, const MCRunArgs &a, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_mc_chunk, dim3(1), dim3(kMCT), 0, st, s, t, mv, a);
+    // the water and argon boxes (cubic, switched) and cubic all-pair runs get
+    // their own instances; other systems run the generic one
+    if (s.pbtype == MCX_PBC_CUBIC && s.cutmode == MCX_CUT_SWITCH)
+        hipLaunchKernelGGL((k_mc_chunk<MCX_PBC_CUBIC, MCX_CUT_SWITCH>), dim3(1), dim3(kMCT), 0, st, s, t, mv, a);
+    else if (s.pbtype == MCX_PBC_CUBIC && s.cutmode == MCX_CUT_FULL)
+        hipLaunchKernelGGL((k_mc_chunk<MCX_PBC_CUBIC, MCX_CUT_FULL>), dim3(1), dim3(kMCT), 0, st, s, t, mv, a);
+    else
+        hipLaunchKernelGGL((k_mc_chunk<-1, -1>), dim3(1), dim3(kMCT), 0, st, s, t, mv, a);
 }
