@@ -30,7 +30,6 @@ namespace {
 #endif
 constexpr int kMCT = MCX_MC_THREADS;
 constexpr int kPairU = MCX_MC_UNROLL;   // list entries in flight per thread
-constexpr int kMaxRec = 96;        // term records of one move staged in LDS
 constexpr int kMCW = kMCT / 64;
 
 // Shared state of one move.
@@ -50,9 +49,6 @@ struct MoveLDS {
     double ms[kMaxMoveAtoms];    // masses (centre-of-mass pivots)
     double xpiv[3];              // pivot coordinates when the pivot is not moved
     int ra0, ra1;                // term-record range of the move (ra1 < 0: use the per-atom adjacency)
-    int4 rid[kMaxRec];
-    int rkk[kMaxRec], rty[kMaxRec];
-    double4 rpar[kMaxRec];
     double red[6][kMCW];
     double de;
     double xi[3];                // force bias: the Gaussian draw
@@ -154,7 +150,7 @@ __device__ __forceinline__ bool set_holds(const MoveLDS &m, const int4 id)
 }
 
 // dE of the move described in m (trial coordinates, displacements and the
-// cached moved-atom data set by make_trial), in three parts: the nonbonded
+// cached moved-atom data set by pick_load and trial_w0), in three parts: the nonbonded
 // pairs of the moved atoms' list rows (threads 64.., each issuing the loads
 // of kPairU list entries together), the bonded and 1-4 terms (wavefront 0,
 // less those among the atoms of a rigidly moved set), and the workgroup
@@ -218,16 +214,14 @@ __device__ __forceinline__ void delta_terms(const DevSys &s, const DevMoves &mv,
     const CoordFetch fo{s.x, s.y, s.z, nullptr, nullptr, nullptr, 0, -1};
     const CoordFetch fn{s.x, s.y, s.z, m.tx, m.ty, m.tz, m.first, m.last};
     if (m.ra1 >= 0) {
-        // self-contained records, staged in LDS by load_moved (or read from
-        // global memory past kMaxRec)
+        // self-contained records of the move instance (their loads overlap
+        // the other wavefronts' pair loop)
         for (int e = m.ra0 + tid; e < m.ra1; e += 64) {
-            const int r = e - m.ra0;
-            const bool lds = r < kMaxRec;
-            const int4 id = lds ? m.rid[r] : mv.rec_ids[e];
+            const int4 id = mv.rec_ids[e];
             if (m.rigid && set_holds(m, id)) continue;
-            const int kk = lds ? m.rkk[r] : mv.rec_kk[e];
-            const int ty = lds ? m.rty[r] : mv.rec_type[e];
-            const double4 par = lds ? m.rpar[r] : mv.rec_par[e];
+            const int kk = mv.rec_kk[e];
+            const int ty = mv.rec_type[e];
+            const double4 par = mv.rec_par[e];
             double e0, c0, v0, e1, c1, v1;
             mcx_term_energy_rec(s, kk >> 28, id, ty, par, fo, e0, c0, v0);
             mcx_term_energy_rec(s, kk >> 28, id, ty, par, fn, e1, c1, v1);
@@ -299,8 +293,8 @@ __device__ __forceinline__ void move_delta(const DevSys &s, const DevTerms &t, c
     delta_finish(m, fb, dec, dev, deb, fnew);
 }
 
-// Phase B: every datum of the moved atoms, the pivot and the move's term
-// records in one round of parallel loads.
+// Every datum of the moved atoms and the pivot, one thread each (the
+// one-shot dE kernel).
 __device__ __forceinline__ void load_moved(const DevSys &s, const DevMoves &mv, MoveLDS &m)
 {
     const int a = threadIdx.x;
@@ -324,14 +318,6 @@ __device__ __forceinline__ void load_moved(const DevSys &s, const DevMoves &mv, 
             m.xpiv[2] = __hip_atomic_load(s.z + pv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
     }
-    const int r = a - 128;
-    if (r >= 0 && r < kMaxRec && m.ra0 + r < m.ra1) {
-        const int e = m.ra0 + r;
-        m.rid[r] = mv.rec_ids[e];
-        m.rkk[r] = mv.rec_kk[e];
-        m.rty[r] = mv.rec_type[e];
-        m.rpar[r] = mv.rec_par[e];
-    }
     __syncthreads();
 }
 
@@ -345,11 +331,15 @@ __device__ __forceinline__ bool beyond_half_skin(const DevSys &s, int4 p, int4 r
     return dx * dx + dy * dy + dz * dz > half_skin2;
 }
 
-// Phase C: trial coordinates (MOVE_TRN::tr_a_b / MOVE_ROT::apply_rotation,
-// include/Move.hpp:42-50,120-143, each atom re-wrapped) and displacements.
-__device__ __forceinline__ void make_trial(const DevSys &s, MoveLDS &m, const int4 *ipos_ref, double half_skin2)
+// Wavefront 0, lane a < nm: trial coordinates (MOVE_TRN::tr_a_b /
+// MOVE_ROT::apply_rotation, include/Move.hpp:42-50,120-143, each atom
+// re-wrapped) and displacements of moved atom a; m.scan and m.rigid from
+// ballots.
+__device__ __forceinline__ void trial_w0(const DevSys &s, MoveLDS &m, int lane, const int4 *ipos_ref,
+                                         double half_skin2)
 {
-    const int a = threadIdx.x;
+    const int a = lane;
+    bool far = false, beyond = false;
     if (a < m.nm) {
         const double x = m.ox[a], y = m.oy[a], z = m.oz[a];
         double ux, uy, uz;
@@ -379,33 +369,105 @@ __device__ __forceinline__ void make_trial(const DevSys &s, MoveLDS &m, const in
             // (include/Move.hpp:120-143), so a set the box edge splits is
             // not rigid and keeps them.
             const double lq = 0.25 * fmin(s.L[0], fmin(s.L[1], s.L[2]));
-            if (s.pbtype != 0 && px * px + py * py + pz * pz >= lq * lq) atomicAnd(&m.rigid, 0);
+            far = s.pbtype != 0 && px * px + py * py + pz * pz >= lq * lq;
             ux = piv[0] + m.R[0][0] * px + m.R[1][0] * py + m.R[2][0] * pz;
             uy = piv[1] + m.R[0][1] * px + m.R[1][1] * py + m.R[2][1] * pz;
             uz = piv[2] + m.R[0][2] * px + m.R[1][2] * py + m.R[2][2] * pz;
         }
         m.dx[a] = ux - x; m.dy[a] = uy - y; m.dz[a] = uz - z;
         m.tx[a] = wrap1(s, ux, 0); m.ty[a] = wrap1(s, uy, 1); m.tz[a] = wrap1(s, uz, 2);
-        if (ipos_ref && beyond_half_skin(s, mcx_fixed4(s, m.tx[a], m.ty[a], m.tz[a], 0.f), ipos_ref[m.first + a],
-                                         half_skin2))
-            atomicOr(&m.scan, 1);
+        beyond = ipos_ref && beyond_half_skin(s, mcx_fixed4(s, m.tx[a], m.ty[a], m.tz[a], 0.f),
+                                              ipos_ref[m.first + a], half_skin2);
     }
-    __syncthreads();
+    const bool scan = __ballot(beyond) != 0ull, rigid = __ballot(far) == 0ull;
+    if (a == 0) { m.scan = scan; m.rigid = rigid; }
+    wave_sync();
 }
 
-// Prefix of the moved atoms' list-row lengths (the flattened pair index of
-// move_delta and force_old).
-__device__ __forceinline__ void row_prefix(const DevSys &s, MoveLDS &m)
+// Wavefront 0: prefix of the moved atoms' list-row lengths (every atom when
+// the trial leaves the list's validity), the flattened pair index of
+// delta_pairs and force_old.
+__device__ __forceinline__ void prefix_w0(const DevSys &s, MoveLDS &m, int lane)
 {
-    if (threadIdx.x == 64) {
-        if (m.scan)
-            for (int b = 0; b < m.nm; b++) m.mc[b] = s.n;
-        int acc = 0;
-        for (int b = 0; b < m.nm; b++) { m.pref[b] = acc; acc += m.mc[b]; }
-        m.pref[m.nm] = acc;
-        m.total = acc;
+    const int nm = m.nm;
+    int c = 0;
+    if (lane < nm) {
+        c = m.scan ? s.n : m.mc[lane];
+        m.mc[lane] = c;
     }
-    __syncthreads();
+    int v = c;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(v, off, 64);
+        if (lane >= off) v += u;
+    }
+    if (lane < nm) m.pref[lane] = v - c;
+    if (lane == nm - 1) { m.pref[nm] = v; m.total = v; }
+    wave_sync();
+}
+
+// Phase A of the chain loop (wavefront 0): draw the move, read the moved
+// atoms and the pivot (their loads issued before the displacement draws, so
+// they are in flight meanwhile), publish everything in m; then, except for
+// a force-bias move (whose trial waits for the force), the trial and the
+// row prefix.  Everything the other wavefronts need is in m at the next
+// workgroup barrier.
+__device__ __forceinline__ void pick_load(const DevSys &s, const DevMoves &mv, WaveDraws &dr, const double *dmax,
+                                          MoveLDS &m, int lane, const int4 *ipos_ref, double half_skin2)
+{
+    int tp, in;
+    int4 rec;
+    int2 rr;
+    pick_instance(mv, dr, lane, tp, in, rec, rr);
+    const int type = mv.type[tp];
+    const int first = rec.x, last = rec.y, nm = last - first + 1, pv = rec.z;
+    const bool mine = lane < nm;
+    const int i = first + (mine ? lane : 0);
+    double ox = 0.0, oy = 0.0, oz = 0.0, ms = 1.0;
+    int4 ip = make_int4(0, 0, 0, 0);
+    double4 mq = make_double4(0.0, 0.0, 0.0, 0.0);
+    int mc = 0;
+    if (mine) {
+        ox = s.x[i]; oy = s.y[i]; oz = s.z[i];
+        ip = s.ipos[i];
+        mq = s.qes[i];
+        mc = s.cutmode == 0 ? s.n : s.cnt[i];
+        if (mv.mass) ms = mv.mass[i];
+    }
+    const bool pout = pv >= 0 && (pv < first || pv > last);
+    double p0 = 0.0, p1 = 0.0, p2 = 0.0;
+    if (pout && lane == 0) {
+        // a uniform address this wavefront wrote with vector stores: atomic
+        // loads, which are never compiled as scalar-cache loads (the scalar
+        // cache does not see those stores)
+        p0 = __hip_atomic_load(s.x + pv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        p1 = __hip_atomic_load(s.y + pv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        p2 = __hip_atomic_load(s.z + pv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    double r[3];
+    double R[3][3];
+    pick_displacement(dr, dmax, tp, type, lane, r, R);
+    if (lane == 0) {
+        m.type = type; m.inst = in; m.acc = tp;
+        m.first = first; m.last = last; m.pivot = pv;
+        m.ra0 = rr.x; m.ra1 = rr.y;
+        m.nm = nm;
+        m.r[0] = r[0]; m.r[1] = r[1]; m.r[2] = r[2];
+        if (type == MCX_MV_ROT_DEV)
+            for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) m.R[a][b] = R[a][b];
+        if (pout) { m.xpiv[0] = p0; m.xpiv[1] = p1; m.xpiv[2] = p2; }
+        m.scan = 0;
+        m.rigid = 1;
+    }
+    if (mine) {
+        m.ox[lane] = ox; m.oy[lane] = oy; m.oz[lane] = oz;
+        m.ip[lane] = ip;
+        m.mq[lane] = mq;
+        m.mc[lane] = mc;
+        m.ms[lane] = ms;
+    }
+    wave_sync();
+    if (type != MCX_MV_FBIAS_DEV) trial_w0(s, m, lane, ipos_ref, half_skin2);
+    prefix_w0(s, m, lane);
 }
 
 // The system with the box type and cutoff mode made compile-time
@@ -458,21 +520,20 @@ __global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s0, DevTerms t, DevMov
     __syncthreads();
 
     // The step loop, in two copies with the same sequence of workgroup
-    // barriers: wavefront 0 runs the chain (pick, trial, bonded terms,
-    // Metropolis test, commit) and the other wavefronts the nonbonded pairs
-    // and the loads they are given.  Separate loops keep each role's values
-    // out of the other's register allocation.
+    // barriers: wavefront 0 runs the chain (pick, moved-atom loads, trial,
+    // bonded terms, Metropolis test, commit) and the other wavefronts the
+    // nonbonded pairs.  Separate loops keep each role's values out of the
+    // other's register allocation.
     int step = 0;
     if (w0) {
         for (; step < arg.nsteps; step++) {
             const int st = arg.step0 + step + 1;
             // ---- phase A (lanes in step): pick and draw the move
-            // (src/MC_metropolis.cpp:85-116; src/MC.cpp:137-156)
-            if (!m.stop) pick_move(mv, dr, dmax, m, lane);
+            // (src/MC_metropolis.cpp:85-116; src/MC.cpp:137-156), read the
+            // moved atoms, form the trial
+            if (!m.stop) pick_load(s, mv, dr, dmax, m, lane, arg.ipos_ref, arg.half_skin2);
             __syncthreads();
             if (m.stop) break;
-            load_moved(s, mv, m);
-            __syncthreads();                  // row_prefix (thread 64)
             const bool fb = m.type == MCX_MV_FBIAS_DEV;
             if (fb) {
                 // force-bias proposal: dx = beta A F + sqrt(2A) xi with F the
@@ -486,10 +547,11 @@ __global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s0, DevTerms t, DevMov
                         m.r[c] = bA * m.fold[c] + sq * m.xi[c];
                     }
                 }
+                wave_sync();
+                trial_w0(s, m, lane, arg.ipos_ref, arg.half_skin2);
+                if (m.scan) prefix_w0(s, m, lane);
                 __syncthreads();
             }
-            make_trial(s, m, arg.ipos_ref, arg.half_skin2);
-            if (m.scan) __syncthreads();      // row_prefix over every atom (thread 64)
             double dec = 0.0, dev = 0.0, deb = 0.0;
             double fnew[3] = {0.0, 0.0, 0.0};
             delta_terms(s, mv, m, dec, dev, deb);
@@ -550,15 +612,11 @@ __global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s0, DevTerms t, DevMov
         for (; step < arg.nsteps; step++) {
             __syncthreads();                  // phase A
             if (m.stop) break;
-            load_moved(s, mv, m);
-            row_prefix(s, m);
             const bool fb = m.type == MCX_MV_FBIAS_DEV;
             if (fb) {
                 force_old(s, m);
-                __syncthreads();
+                __syncthreads();              // the force-bias trial (wavefront 0)
             }
-            __syncthreads();                  // make_trial (wavefront 0)
-            if (m.scan) row_prefix(s, m);     // every atom is a candidate of this trial
             double dec = 0.0, dev = 0.0;
             double fnew[3] = {0.0, 0.0, 0.0};
             delta_pairs(s, m, fb, dec, dev, fnew);

@@ -210,21 +210,24 @@ __device__ __forceinline__ double wrap1(const DevSys &s, double v, int k)
 }
 
 // Phase A of a move (src/MC_metropolis.cpp:85-116; src/MC.cpp:137-156), run
-// by wavefront 0 with its lanes in step: move type, instance, the rndSphere
+// by wavefront 0 with its lanes in step, in two parts.  pick_instance draws
+// the move type and instance and reads the instance record (atom range,
+// pivot, term-record range); pick_displacement draws the rndSphere
 // direction (rejection in the unit ball) and, for rotations, the angle and
 // the rotation matrix.  A force-bias move draws a Gaussian vector instead
-// (Box-Muller; left in m.r for the drift to be added).  Lane 0 publishes the
-// move in m.
-template <class M>
-__device__ __forceinline__ void pick_move(const DevMoves &mv, WaveDraws &dr, const double *dmax, M &m, int lane)
+// (Box-Muller; left in r for the drift to be added).
+__device__ __forceinline__ void pick_instance(const DevMoves &mv, WaveDraws &dr, int lane, int &tp, int &in,
+                                              int4 &rec, int2 &rr)
 {
-    const int tp = (int)((double)mv.ntypes * dr.next(lane));
-    const int in = (int)((double)mv.ninst[tp] * dr.next(lane));
-    const int4 rec = mv.inst[mv.off[tp] + in];
-    const int2 rr = mv.inst_adj[mv.off[tp] + in];
-    const int type = mv.type[tp];
-    double r[3];
-    double R[3][3];
+    tp = (int)((double)mv.ntypes * dr.next(lane));
+    in = (int)((double)mv.ninst[tp] * dr.next(lane));
+    rec = mv.inst[mv.off[tp] + in];
+    rr = mv.inst_adj[mv.off[tp] + in];
+}
+
+__device__ __forceinline__ void pick_displacement(WaveDraws &dr, const double *dmax, int tp, int type, int lane,
+                                                  double (&r)[3], double (&R)[3][3])
+{
     if (type == MCX_MV_FBIAS_DEV) {
         // force bias: three standard normals by Box-Muller from four draws
         const double u1 = dr.next(lane), u2 = dr.next(lane), u3 = dr.next(lane), u4 = dr.next(lane);
@@ -245,6 +248,20 @@ __device__ __forceinline__ void pick_move(const DevMoves &mv, WaveDraws &dr, con
         const double rang = dmax[tp] * (dr.next(lane) * 2.0 + (-1.0));
         rot_mat(R, r, rang);
     }
+}
+
+// Both parts; lane 0 publishes the move in m.
+template <class M>
+__device__ __forceinline__ void pick_move(const DevMoves &mv, WaveDraws &dr, const double *dmax, M &m, int lane)
+{
+    int tp, in;
+    int4 rec;
+    int2 rr;
+    pick_instance(mv, dr, lane, tp, in, rec, rr);
+    const int type = mv.type[tp];
+    double r[3];
+    double R[3][3];
+    pick_displacement(dr, dmax, tp, type, lane, r, R);
     if (lane == 0) {
         m.type = type; m.inst = in; m.acc = tp;
         m.first = rec.x; m.last = rec.y; m.pivot = rec.z;
