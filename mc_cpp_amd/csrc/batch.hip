@@ -228,7 +228,7 @@ __global__ __launch_bounds__(NT) void k_batch_sweep(BatchArgs a)
     __shared__ long long n_trial[kMaxMoveTypes], n_acc[kMaxMoveTypes];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const bool w0 = tid < 64;
+    const bool w0 = __builtin_amdgcn_readfirstlane(tid) < 64;   // wave-uniform
     const int r = a.order[blockIdx.x];
     const BatchRep &R = a.reps[r];
     const DevSys s = R.s;            // fields used in the pair loops: kept in registers
@@ -272,14 +272,18 @@ __global__ __launch_bounds__(NT) void k_batch_sweep(BatchArgs a)
     const int nmoves = a.nmoves > 0 ? a.nmoves : R.sweep_len;
     unsigned npair = 0;     // this lane's in-cutoff pair evaluations (old and new) of the scan
 
-    for (int mi = 0; mi < nmoves; mi++) {
-        // ---- wavefront 0: the move (src/MC_metropolis.cpp:85-116;
-        // src/MC.cpp:137-156), the moved atoms and the trial coordinates
-        // (MOVE_TRN::tr_a_b / MOVE_ROT::apply_rotation,
-        // include/Move.hpp:42-50,120-143), with no barrier in between
-        int tp = 0, first = 0, last = 0, nm = 0, ra0 = 0, ra1 = 0, pa0 = 0, pa1 = 0, inst = 0;
-        bool rigid = false;
-        if (w0) {
+    // The move loop in two copies with the same barriers: wavefront 0 runs
+    // the chain (move, trial, bonded terms, Metropolis test, commit) and
+    // the other wavefronts the pair scan, so that each role's values stay
+    // out of the other's register allocation.
+    if (w0) {
+        for (int mi = 0; mi < nmoves; mi++) {
+            // ---- wavefront 0: the move (src/MC_metropolis.cpp:85-116;
+            // src/MC.cpp:137-156), the moved atoms and the trial coordinates
+            // (MOVE_TRN::tr_a_b / MOVE_ROT::apply_rotation,
+            // include/Move.hpp:42-50,120-143), with no barrier in between
+            int tp = 0, first = 0, last = 0, nm = 0, ra0 = 0, ra1 = 0, pa0 = 0, pa1 = 0, inst = 0;
+            bool rigid = false;
             tp = (int)((double)tab.ntypes * dr.next(lane));
             inst = (int)((double)tab.ninst[tp] * dr.next(lane));
             const int k = tab.off[tp] + inst;
@@ -323,7 +327,12 @@ __global__ __launch_bounds__(NT) void k_batch_sweep(BatchArgs a)
             double piv[3] = {0.0, 0.0, 0.0};
             if (type != MCX_MV_TRN_DEV) {
                 if (pivot >= 0 && (pivot < first || pivot > last)) {
-                    piv[0] = s.x[pivot]; piv[1] = s.y[pivot]; piv[2] = s.z[pivot];
+                    // a uniform address this wavefront writes with vector
+                    // stores: atomic loads are never compiled as scalar-cache
+                    // loads, which would not see those stores
+                    piv[0] = __hip_atomic_load(s.x + pivot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    piv[1] = __hip_atomic_load(s.y + pivot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    piv[2] = __hip_atomic_load(s.z + pivot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 } else {
                     wave_sync();
                     if (pivot >= 0) {
@@ -388,13 +397,11 @@ __global__ __launch_bounds__(NT) void k_batch_sweep(BatchArgs a)
                           !(reach * 1.00001 < 0.5 * s.L[0] && reach * 1.00001 < 0.5 * s.L[1] &&
                             reach * 1.00001 < 0.5 * s.L[2]);
             }
-        }
-        __syncthreads();
-        // ---- dE: wavefronts 1.. scan the nonbonded pairs; wavefront 0
-        // takes the bonded and 1-4 terms, removes excluded pairs the scan
-        // counted, and evaluates pairs of two moved atoms
-        double part = 0.0;
-        if (w0) {
+            __syncthreads();
+            // ---- dE: wavefronts 1.. scan the nonbonded pairs; wavefront 0
+            // takes the bonded and 1-4 terms, removes excluded pairs the scan
+            // counted, and evaluates pairs of two moved atoms
+            double part = 0.0;
             double dec = 0.0, dev = 0.0, deb = 0.0;
             // moved atoms' old and trial coordinates from LDS, others from memory
             const CoordFetch fo{s.x, s.y, s.z, m.ox, m.oy, m.oz, first, last};
@@ -446,19 +453,12 @@ __global__ __launch_bounds__(NT) void k_batch_sweep(BatchArgs a)
                 }
             }
             part = MCX_CHGCHARMM * MCX_KCALTOIU * dec + 4.0 * dev + deb;
-        } else {
-            const int n_m = m.nm;
-            if (n_m == 3) part = scan_pairs<NT, U, IMG, 3, QT>(s, m, f, ptab, queue, frozen_any, npair);
-            else if (n_m == 1) part = scan_pairs<NT, U, IMG, 1, QT>(s, m, f, ptab, queue, frozen_any, npair);
-            else part = scan_pairs<NT, U, IMG, 0, QT>(s, m, f, ptab, queue, frozen_any, npair);
-        }
-        part = wave_total(part);
-        if (lane == 0) m.red[tid >> 6] = part;
-        __syncthreads();
-        // ---- wavefront 0: Metropolis test (apply_criterion,
-        // src/MC_metropolis.cpp:175-204), bookkeeping and commit; the next
-        // move's barrier publishes the commit
-        if (w0) {
+            part = wave_total(part);
+            if (lane == 0) m.red[tid >> 6] = part;
+            __syncthreads();
+            // ---- wavefront 0: Metropolis test (apply_criterion,
+            // src/MC_metropolis.cpp:175-204), bookkeeping and commit; the next
+            // move's barrier publishes the commit
             const double de = wave_total(lane < NW ? m.red[lane] : 0.0);
             const int acc = metropolis(dr, beta, de, lane);
             if (lane == 0) {
@@ -490,6 +490,18 @@ __global__ __launch_bounds__(NT) void k_batch_sweep(BatchArgs a)
                 s.ipos[i] = p;
                 if (IMG) img[i] = make_int4(p.x, p.y, p.z, tw);
             }
+        }
+    } else {
+        for (int mi = 0; mi < nmoves; mi++) {
+            __syncthreads();              // the move is published
+            double part = 0.0;
+            const int n_m = m.nm;
+            if (n_m == 3) part = scan_pairs<NT, U, IMG, 3, QT>(s, m, f, ptab, queue, frozen_any, npair);
+            else if (n_m == 1) part = scan_pairs<NT, U, IMG, 1, QT>(s, m, f, ptab, queue, frozen_any, npair);
+            else part = scan_pairs<NT, U, IMG, 0, QT>(s, m, f, ptab, queue, frozen_any, npair);
+            part = wave_total(part);
+            if (lane == 0) m.red[tid >> 6] = part;
+            __syncthreads();
         }
     }
     // pair evaluations of the launch: one atomic per wavefront
