@@ -271,6 +271,8 @@ __device__ __forceinline__ int wave_incl_scan(int v)
 // Ordered compaction of row i's bitmap B (LDS) into its ELL row; sets cnt,
 // half (entries below i) and the overflow high-water mark.  One wavefront,
 // no barriers.
+constexpr int kSparseBits = 8;     // words with more set bits are written cooperatively
+
 __device__ void compact_row(const DevSys &s, const uint32_t *B, int words, int i, int lane)
 {
     int *row = s.nbr + (size_t)i * s.cap;
@@ -286,11 +288,34 @@ __device__ void compact_row(const DevSys &s, const uint32_t *B, int words, int i
             const int below = pos + __popc(wd & ((1u << (i & 31)) - 1u));
             nlow = __builtin_amdgcn_readlane(below, iw - base);
         }
-        while (wd) {
-            const int b = __ffs(wd) - 1;
-            wd &= wd - 1u;
-            if (pos < s.cap) row[pos] = k * 32 + b;
-            pos++;
+        // Sparse words (at most kSparseBits set): each lane writes its own.
+        // Denser words -- a neighbour row sets runs of consecutive indices --
+        // go two per step with 32 lanes each, one bit per lane, so the loop
+        // count no longer follows the densest word of the chunk.
+        const bool dense = pc > kSparseBits;
+        if (!dense) {
+            while (wd) {
+                const int b = __ffs(wd) - 1;
+                wd &= wd - 1u;
+                if (pos < s.cap) row[pos] = k * 32 + b;
+                pos++;
+            }
+        }
+        uint64_t dm = __ballot(dense);
+        while (dm) {
+            const int l0 = __builtin_ctzll(dm);
+            dm &= dm - 1;
+            const int l1 = dm ? __builtin_ctzll(dm) : -1;
+            if (dm) dm &= dm - 1;
+            const int src = lane < 32 ? l0 : l1;
+            const int from = src < 0 ? l0 : src;
+            const uint32_t w = (uint32_t)__shfl((int)wd, from, 64);
+            const int p0 = __shfl(pos, from, 64);
+            const int b = lane & 31;
+            if (src >= 0 && ((w >> b) & 1u)) {
+                const int p = p0 + __popc(w & ((1u << b) - 1u));
+                if (p < s.cap) row[p] = (base + src) * 32 + b;
+            }
         }
         total += __builtin_amdgcn_readlane(incl, 63);
     }
