@@ -11,6 +11,8 @@ Tolerances (stated per quantity):
     whose dE lies within the dE tolerance of the acceptance threshold
     (after such a move the two chains legitimately diverge).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -255,6 +257,42 @@ def test_mc_replay_small_box():
     # device's fresh total agrees with the FP64 one
     assert abs(res["etot"] - ro["etot"]) <= 1e-6 * max(e0, abs(ro["etot"]))
     assert abs(ff.getE() - Oracle(ref).getE()[0]) <= 1e-6 * max(e0, abs(ro["etot"]))
+
+
+def _replay_system(kind):
+    if kind == "cubic_full":
+        s, _ = systems.water_box(seed=5, nside=5, L=15.53, nmol=125, tag=41)
+        s.cutmode = systems.CUT_FULL
+    elif kind == "orthorhombic_switch":
+        s, _ = systems.argon_fcc(seed=13)
+        s.pbtype = systems.PBC_ORBIC
+        s.box = np.array([34.39, 36.1, 33.2])
+        s.x, s.y, s.z = systems.wrap(s, s.x, s.y, s.z)
+    else:   # vacuum_full: a flexible dipeptide, no periodic box, atom moves
+        from mc_cpp_amd import deck
+        s = deck.read_deck(os.path.join(os.path.dirname(__file__), "golden", "decks", "diala_vac.xml")).system
+        s.pbtype = systems.PBC_NONE
+    return s
+
+
+@pytest.mark.parametrize("kind", ["cubic_full", "orthorhombic_switch", "vacuum_full"])
+def test_mc_replay_box_and_cutoff_kinds(kind):
+    """The chain kernel is compiled per box type and cutoff mode (cubic
+    switched, cubic all-pair, and a generic instance): replay each against
+    the FP64 oracle."""
+    s = _replay_system(kind)
+    o, ref = _pair(s)
+    if s.cutmode == systems.CUT_SWITCH:
+        o.update_list()
+    e0 = abs(o.getE()[0])
+    ff = mcx.FField(s)
+    n = 1500
+    res = mcx.MC_metropolis(ff, nsteps=n, seed=31).run()
+    ro = o.mc_run(n, 31, mode=1)
+    kT = _kT(s)
+    n_ok = _compare_replay(res, ro, None, lambda de: 1e-5 * max(abs(de), kT))
+    assert n_ok == n
+    assert abs(res["etot"] - ro["etot"]) <= 1e-6 * max(e0, abs(ro["etot"]))
 
 
 def test_gradient_vs_fp64_and_fd():
