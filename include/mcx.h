@@ -117,6 +117,14 @@ int mcx_update_list(mcx_ctx *h, long long *npairs);
  * list, row i holding ascending j > i, written back to back into `flat`. */
 int mcx_get_list(mcx_ctx *h, int *neighPair, int *flat, long long cap);
 
+/* Row capacity of the pair list (entries per atom), the sizeList of
+ * List_nonBonded::init_verlet_list (src/List_nonBonded.cpp:827-830, sized
+ * there as max_i neighPair * 1.04 + 1 at the first build).  set > 0 makes
+ * it exactly `set` and marks the list stale; set == 0 leaves it.  *current
+ * (if non-null) receives the capacity in use.  Unlike the reference, a build
+ * that needs more -- also one inside mcx_mc_run -- grows it and rebuilds. */
+int mcx_list_capacity(mcx_ctx *h, int set, int *current);
+
 /* FField::getE(double ener[10]) (src/FField_MDBAS.cpp:77-106):
  * {tot, pot, kin, elec, vdw, bond, ang, ub, dihe, impr} in iu. */
 int mcx_energy(mcx_ctx *h, double ener[10]);

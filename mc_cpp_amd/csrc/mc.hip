@@ -19,6 +19,7 @@
 #include "mcmove.h"
 
 #include <cfloat>
+#include <climits>
 
 namespace {
 
@@ -492,6 +493,12 @@ template <int PB, int CUT>
 __global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s0, DevTerms t, DevMoves mv, MCRunArgs arg)
 {
     const DevSys s = specialise<PB, CUT>(s0);
+    if (arg.bail && (s.overflow[0] != 0 || arg.bail[0] != INT_MAX)) {
+        // the list this chunk would use is incomplete: leave every state
+        // untouched; the host grows the list and runs again from here
+        if (threadIdx.x == 0) atomicMin(arg.bail, arg.step0);
+        return;
+    }
     __shared__ MoveLDS m;
     __shared__ uint32_t mt[kMTN];
     __shared__ double dmax[kMaxMoveTypes];

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -810,6 +811,22 @@ int mcx_update_list(mcx_ctx *h, long long *npairs)
     });
 }
 
+int mcx_list_capacity(mcx_ctx *h, int set, int *current)
+{
+    return guarded(h, [&] {
+        if (set < 0) throw mcx_error(MCX_ERR_ARG, "capacity < 0");
+        if (set > 0) {
+            h->cap = set;
+            h->nbr.alloc((size_t)h->n * set);
+            h->cnt.alloc(h->n);
+            h->half.alloc(h->n);
+            h->overflow.alloc(1);
+            h->list_valid = false;
+        }
+        if (current) *current = h->cap;
+    });
+}
+
 int mcx_get_list(mcx_ctx *h, int *neighPair, int *flat, long long cap)
 {
     return guarded(h, [&] {
@@ -892,14 +909,14 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
         DBuf<double> de_d;
         DBuf<int> typ_d, inst_d;
         DBuf<double> disp_d, lnq_d;
-        DBuf<int> stop_d;
+        DBuf<int> stop_d, bail_d;
         struct Cleanup {
             std::vector<std::function<void()>> f;
             ~Cleanup() { for (auto &g : f) g(); }
         } cleanup;
         cleanup.f.push_back([&] { etot.release(); dmax.release(); mts.release(); trials.release(); accepted.release();
                                   acc_tmp.release(); acc_d.release(); de_d.release(); typ_d.release(); inst_d.release();
-                                  disp_d.release(); lnq_d.release(); stop_d.release(); });
+                                  disp_d.release(); lnq_d.release(); stop_d.release(); bail_d.release(); });
         double ener[10], r9[9];
         energy_device(h, r9);
         assemble(r9, ener);
@@ -980,7 +997,33 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
             }
             if (save && done % save == 0) emit_frame(done);
         }
-        const int every = (list_every > 0 && h->cutmode != MCX_CUT_FULL) ? list_every : nsteps;
+        const bool rebuilds = list_every > 0 && h->cutmode != MCX_CUT_FULL;
+        const int every = rebuilds ? list_every : nsteps;
+        // Chunks and list rebuilds are queued without waiting on the host.
+        // A rebuild that overflows the list capacity makes the following
+        // chunks return untouched (the bail-out of k_mc_chunk); every
+        // kCheckEvery chunks, and before a frame or the end, the host looks:
+        // it then grows the list, rebuilds it and resumes at the first chunk
+        // that bailed out.
+        constexpr int kCheckEvery = 16;
+        const int kNone = INT_MAX;
+        if (!autolist && rebuilds) {
+            bail_d.upload(&kNone, 1, h->stream);
+            a.bail = bail_d.p;
+        }
+        int since_check = 0;
+        auto check_bail = [&]() {
+            if (!a.bail) return;
+            int b = kNone;
+            HIPCHK(hipMemcpyAsync(&b, bail_d.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+            if (b != kNone) {
+                done = b;
+                build_list(h);               // grows the capacity until the rows fit
+                bail_d.upload(&kNone, 1, h->stream);
+            }
+            since_check = 0;
+        };
         while (!autolist && done < nsteps) {
             int chunk = every - (done % std::max(every, 1));
             if (save) chunk = std::min(chunk, save - done % save);
@@ -990,12 +1033,15 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
             launch_mc_chunk(h->sys(), h->dterms(), mv, a, h->stream);
             HIPCHK(hipGetLastError());
             done += chunk;
-            if (list_every > 0 && h->cutmode != MCX_CUT_FULL && done % list_every == 0) {
-                rebuild_list_fast(h);
-                nlist++;
+            if (rebuilds && done % list_every == 0) {
+                launch_list_build(h->sys(), h->grid, h->stream);
+                HIPCHK(hipGetLastError());
             }
-            if (save && done % save == 0) emit_frame(done);
+            const bool frame = save && done % save == 0;
+            if (frame || ++since_check == kCheckEvery || done >= nsteps) check_bail();
+            if (frame && done % save == 0) emit_frame(done);
         }
+        if (rebuilds) nlist += nsteps / list_every;
         HIPCHK(hipStreamSynchronize(h->stream));
         const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (stats) {

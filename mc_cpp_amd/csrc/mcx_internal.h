@@ -192,6 +192,10 @@ struct MCRunArgs {
     const int4 *ipos_ref;  // null: fixed rebuild schedule
     double half_skin2;     // (dcut/2)^2, A^2
     int *stop_at;          // device scalar
+    // fixed rebuild schedule, list rebuilds queued without a host wait: a
+    // chunk that finds the preceding rebuild overflowed (s.overflow) or an
+    // earlier chunk bailed out changes nothing and records its first step
+    int *bail;             // device scalar (INT_MAX: none), or null
 };
 void launch_mc_chunk(const DevSys &s, const DevTerms &t, const DevMoves &mv, const MCRunArgs &a,
                      hipStream_t st);

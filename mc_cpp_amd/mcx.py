@@ -86,6 +86,7 @@ SIGNATURES = {
     "mcx_get_coords": ([C.c_void_p, _dp, _dp, _dp], C.c_int),
     "mcx_update_list": ([C.c_void_p, C.POINTER(C.c_longlong)], C.c_int),
     "mcx_get_list": ([C.c_void_p, _ip, _ip, C.c_longlong], C.c_int),
+    "mcx_list_capacity": ([C.c_void_p, C.c_int, _ip], C.c_int),
     "mcx_energy": ([C.c_void_p, _dp], C.c_int),
     "mcx_delta_energy": ([C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp], C.c_int),
     "mcx_mc_run": ([C.c_void_p, C.POINTER(MoveTypeC), C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int,
@@ -230,6 +231,13 @@ class Handle:
         flat = np.zeros(max(int(cnt.sum()), 1), np.int32)
         self._chk(self.L.mcx_get_list(self.h, _p(cnt, C.c_int), _p(flat, C.c_int), flat.size))
         return cnt, flat[: int(cnt.sum())]
+
+    def list_capacity(self, set_to: int = 0) -> int:
+        """Row capacity of the pair list; set_to > 0 makes it exactly that
+        (List_nonBonded's sizeList) and marks the list stale."""
+        cur = C.c_int(0)
+        self._chk(self.L.mcx_list_capacity(self.h, int(set_to), C.byref(cur)))
+        return cur.value
 
     def get_exclusions(self):
         ep = np.zeros(self.n, np.int32)

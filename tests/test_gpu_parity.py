@@ -295,6 +295,31 @@ def test_mc_replay_box_and_cutoff_kinds(kind):
     assert abs(res["etot"] - ro["etot"]) <= 1e-6 * max(e0, abs(ro["etot"]))
 
 
+def test_mc_replay_through_list_overflow():
+    """A list capacity equal to the largest row of the first build (the
+    reference's sizeList is max_i neighPair * 1.04 + 1): rows grow during
+    the run, a scheduled rebuild inside mcx_mc_run overflows, the chunks
+    queued after it leave the chain untouched, and the library grows the
+    list and resumes there.  The chain still equals the oracle's."""
+    s, _ = systems.water_box(seed=3, nside=8, L=24.85)
+    o, ref = _pair(s)
+    o.update_list()
+    cnt, rows = o.list_rows()
+    full = cnt.astype(np.int64).copy()        # full row = half row + the rows naming i
+    for r in rows:
+        np.add.at(full, r, 1)
+    m0 = int(full.max())
+    ff = mcx.FField(s)
+    assert ff.h.list_capacity(m0) == m0
+    n = 2000
+    res = mcx.MC_metropolis(ff, nsteps=n, seed=777).run()
+    ro = o.mc_run(n, 777, mode=1)
+    kT = _kT(s)
+    n_ok = _compare_replay(res, ro, None, lambda de: 1e-5 * max(abs(de), kT))
+    assert n_ok == n
+    assert ff.h.list_capacity() > m0, "no rebuild outgrew the capacity: the overflow path was not exercised"
+
+
 def test_gradient_vs_fp64_and_fd():
     s, _ = systems.water_box(seed=3, nside=8, L=24.85)
     o, ref = _pair(s)
