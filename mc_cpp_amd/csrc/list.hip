@@ -281,6 +281,10 @@ __device__ void compact_row(const DevSys &s, const uint32_t *B, int words, int i
     for (int base = 0; base < words; base += 64) {
         const int k = base + lane;
         uint32_t wd = k < words ? B[k] : 0u;
+        if (__ballot(wd != 0u) == 0ull) {          // an empty chunk (most of them in a large box)
+            if (iw >= base && iw < base + 64) nlow = total;
+            continue;
+        }
         const int pc = __popc(wd);
         const int incl = wave_incl_scan(pc);
         int pos = total + incl - pc;
