@@ -364,8 +364,8 @@ __device__ __forceinline__ bool mcx_pair_change(const DevSys &s, double ox, doub
     const double s2 = sij * sij;
     const double to = s2 * (rio * rio), tn = s2 * (rin * rin);
     const double t3o = to * to * to, t3n = tn * tn * tn;
-    const double dc = qj * (Sn * rin - So * rio);
-    const double dv = ej * (Sn * fma(t3n, t3n, -t3n) - So * fma(t3o, t3o, -t3o));
+    const double dc = qj * fma(Sn, rin, -(So * rio));
+    const double dv = ej * fma(Sn, fma(t3n, t3n, -t3n), -(So * fma(t3o, t3o, -t3o)));
     c = qi * dc;
     v = ei * dv;
     return true;

@@ -172,7 +172,8 @@ __device__ __forceinline__ double mcx_switch(const DevSys &s, double r2)
 {
     if (r2 <= s.cton2) return 1.0;
     const double a = s.ctoff2 - r2;
-    return a * a * (s.ctoff2 + 2.0 * r2 - 3.0 * s.cton2) * s.switch2;
+    const double b = fma(2.0, r2, s.ctoff2 - 3.0 * s.cton2);    // ctoff2 + 2 r2 - 3 cton2
+    return (a * a) * (b * s.switch2);
 }
 
 struct CoordFetch {
