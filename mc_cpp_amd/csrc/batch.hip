@@ -23,6 +23,7 @@
 #include "../../include/mcx.h"
 #include "mcmove.h"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace {
@@ -213,7 +214,7 @@ __device__ __forceinline__ double scan_pairs(const DevSys &s, const BMove &m, co
     return MCX_CHGCHARMM * MCX_KCALTOIU * dec + 4.0 * dev;
 }
 
-template <int NT, int U, bool IMG>
+template <int NT, int U, bool IMG, bool CS>
 __global__ __launch_bounds__(NT) void k_batch_sweep(BatchArgs a)
 {
     using QT = typename std::conditional<IMG, unsigned short, int>::type;
@@ -231,7 +232,9 @@ __global__ __launch_bounds__(NT) void k_batch_sweep(BatchArgs a)
     const bool w0 = __builtin_amdgcn_readfirstlane(tid) < 64;   // wave-uniform
     const int r = a.order[blockIdx.x];
     const BatchRep &R = a.reps[r];
-    const DevSys s = R.s;            // fields used in the pair loops: kept in registers
+    // fields used in the pair loops: kept in registers; with CS the box type
+    // and cutoff mode are compile-time constants
+    const DevSys s = CS ? specialise<MCX_PBC_CUBIC, MCX_CUT_SWITCH>(R.s) : R.s;
     const DevMoves &mv = R.mv;
     int4 *img = dyn;
     QT *queue = reinterpret_cast<QT *>(dyn + (IMG ? a.img_cap : 0));
@@ -555,16 +558,18 @@ __global__ __launch_bounds__(256) void k_exchange(ExchArgs x)
 
 using BatchKernel = void (*)(BatchArgs);
 
-template <int NT, int U, bool IMG>
+template <int NT, int U, bool IMG, bool CS>
 constexpr BatchKernel kvar()
 {
-    return &k_batch_sweep<NT, U, IMG>;
+    return &k_batch_sweep<NT, U, IMG, CS>;
 }
 
-BatchKernel variant(int threads, int unroll, bool img)
+BatchKernel variant(int threads, int unroll, bool img, bool cs = false)
 {
-#define MCX_BV(NT, U)                                                     \
-    if (threads == NT && unroll == U) return img ? kvar<NT, U, true>() : kvar<NT, U, false>();
+#define MCX_BV(NT, U)                                                                           \
+    if (threads == NT && unroll == U)                                                           \
+        return img ? (cs ? kvar<NT, U, true, true>() : kvar<NT, U, true, false>())              \
+                   : (cs ? kvar<NT, U, false, true>() : kvar<NT, U, false, false>());
     MCX_BV(256, 1) MCX_BV(256, 2) MCX_BV(256, 4)
     MCX_BV(512, 1) MCX_BV(512, 2) MCX_BV(512, 4)
     MCX_BV(1024, 1) MCX_BV(1024, 2) MCX_BV(1024, 4)
@@ -578,11 +583,15 @@ bool batch_variant_ok(int threads, int unroll, bool img) { return variant(thread
 
 size_t batch_static_lds(int threads, int unroll, bool img)
 {
-    BatchKernel k = variant(threads, unroll, img);
-    if (!k) throw mcx_error(MCX_ERR_ARG, "no batched kernel for this thread count / unroll");
-    hipFuncAttributes at{};
-    HIPCHK(hipFuncGetAttributes(&at, reinterpret_cast<const void *>(k)));
-    return at.sharedSizeBytes;
+    size_t most = 0;
+    for (int cs = 0; cs < 2; cs++) {
+        BatchKernel k = variant(threads, unroll, img, cs != 0);
+        if (!k) throw mcx_error(MCX_ERR_ARG, "no batched kernel for this thread count / unroll");
+        hipFuncAttributes at{};
+        HIPCHK(hipFuncGetAttributes(&at, reinterpret_cast<const void *>(k)));
+        most = std::max(most, (size_t)at.sharedSizeBytes);
+    }
+    return most;
 }
 
 size_t batch_dynamic_lds(int threads, bool img, int img_cap)
@@ -593,15 +602,17 @@ size_t batch_dynamic_lds(int threads, bool img, int img_cap)
 
 void batch_prepare(int threads, int unroll, bool img, int img_cap)
 {
-    BatchKernel k = variant(threads, unroll, img);
-    if (!k) throw mcx_error(MCX_ERR_ARG, "no batched kernel for this thread count / unroll");
-    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)batch_dynamic_lds(threads, img, img_cap)));
+    for (int cs = 0; cs < 2; cs++) {
+        BatchKernel k = variant(threads, unroll, img, cs != 0);
+        if (!k) throw mcx_error(MCX_ERR_ARG, "no batched kernel for this thread count / unroll");
+        HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)batch_dynamic_lds(threads, img, img_cap)));
+    }
 }
 
 void launch_batch_sweep(const BatchArgs &a, int nrep, int threads, int unroll, bool img, hipStream_t st)
 {
-    BatchKernel k = variant(threads, unroll, img);
+    BatchKernel k = variant(threads, unroll, img, a.cs != 0);
     if (!k) throw mcx_error(MCX_ERR_ARG, "no batched kernel for this thread count / unroll");
     hipLaunchKernelGGL(k, dim3(nrep), dim3(threads), batch_dynamic_lds(threads, img, a.img_cap), st, a);
 }

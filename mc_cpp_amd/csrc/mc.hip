@@ -471,24 +471,6 @@ __device__ __forceinline__ void pick_load(const DevSys &s, const DevMoves &mv, W
     prefix_w0(s, m, lane);
 }
 
-// The system with the box type and cutoff mode made compile-time
-// constants (PB, CUT >= 0): the branches on them fold away, and a cubic
-// box's three edges become one value.  PB or CUT < 0 keeps the field.
-template <int PB, int CUT>
-__device__ __forceinline__ DevSys specialise(const DevSys &s0)
-{
-    DevSys s = s0;
-    if (PB >= 0) s.pbtype = PB;
-    if (CUT >= 0) s.cutmode = CUT;
-    if (PB == MCX_PBC_CUBIC) {
-        for (int d = 1; d < 3; d++) {
-            s.L[d] = s0.L[0]; s.rL[d] = s0.rL[0]; s.unit[d] = s0.unit[0];
-            s.Lf[d] = s0.Lf[0]; s.rLf[d] = s0.rLf[0];
-        }
-    }
-    return s;
-}
-
 template <int PB, int CUT>
 __global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s0, DevTerms t, DevMoves mv, MCRunArgs arg)
 {

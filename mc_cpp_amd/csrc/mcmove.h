@@ -209,6 +209,24 @@ __device__ __forceinline__ double wrap1(const DevSys &s, double v, int k)
     return s.pbtype != 0 ? v - s.L[k] * mcx_ref_rint(s.rL[k] * v) : v;
 }
 
+// The system with the box type and cutoff mode made compile-time
+// constants (PB, CUT >= 0): the branches on them fold away, and a cubic
+// box's three edges become one value.  PB or CUT < 0 keeps the field.
+template <int PB, int CUT>
+__device__ __forceinline__ DevSys specialise(const DevSys &s0)
+{
+    DevSys s = s0;
+    if (PB >= 0) s.pbtype = PB;
+    if (CUT >= 0) s.cutmode = CUT;
+    if (PB == MCX_PBC_CUBIC) {
+        for (int d = 1; d < 3; d++) {
+            s.L[d] = s0.L[0]; s.rL[d] = s0.rL[0]; s.unit[d] = s0.unit[0];
+            s.Lf[d] = s0.Lf[0]; s.rLf[d] = s0.rLf[0];
+        }
+    }
+    return s;
+}
+
 // Phase A of a move (src/MC_metropolis.cpp:85-116; src/MC.cpp:137-156), run
 // by wavefront 0 with its lanes in step, in two parts.  pick_instance draws
 // the move type and instance and reads the instance record (atom range,

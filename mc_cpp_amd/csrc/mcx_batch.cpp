@@ -58,6 +58,9 @@ struct mcx_batch {
         a.rec_len = rec_len;
         a.acc_out = acc_out;
         a.de_out = de_out;
+        a.cs = 1;
+        for (const mcx_ctx *h : reps)
+            if (h->pbtype != MCX_PBC_CUBIC || h->cutmode != MCX_CUT_SWITCH) a.cs = 0;
         return a;
     }
     ExchArgs xargs() const

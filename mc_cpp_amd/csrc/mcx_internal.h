@@ -235,6 +235,7 @@ struct BatchArgs {
     int rec_len;                 // moves recorded per replica (the launch's first rec_len)
     unsigned char *acc_out;      // [nrep * rec_len] or null
     double *de_out;              // [nrep * rec_len] or null
+    int cs;                      // every replica cubic with a switched cutoff: the specialised kernel
 };
 struct ExchArgs {
     int nrep;
