@@ -412,7 +412,8 @@ __device__ __forceinline__ void prefix_w0(const DevSys &s, MoveLDS &m, int lane)
 // a force-bias move (whose trial waits for the force), the trial and the
 // row prefix.  Everything the other wavefronts need is in m at the next
 // workgroup barrier.
-__device__ __forceinline__ void pick_load(const DevSys &s, const DevMoves &mv, WaveDraws &dr, const double *dmax,
+template <class D>
+__device__ __forceinline__ void pick_load(const DevSys &s, const DevMoves &mv, D &dr, const double *dmax,
                                           MoveLDS &m, int lane, const int4 *ipos_ref, double half_skin2)
 {
     int tp, in;
@@ -629,6 +630,245 @@ __global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s0, DevTerms t, DevMov
         for (int k = tid; k < kMTN; k += kMCT) arg.mt_state[k] = mt[k];
 }
 
+// ---------------------------------------------------------------------------
+// Read-ahead proposals.  MC_metropolis::apply_criterion draws its uniform
+// only when dE exceeds DBL_EPSILON (src/MC_metropolis.cpp:180-187), so a
+// rejected move always consumed exactly its proposal draws and one
+// acceptance draw, and left the state unchanged: the next move's proposal
+// then starts at a draw position known in advance and sees the same
+// coordinates.  k_mc_spec uses this to take phase A off the critical path:
+// while the other wavefronts evaluate move k's pairs, wavefront 0 draws and
+// loads move k+1 into a second buffer on the assumption that move k is
+// rejected (about four moves in five).  If it is, the next step starts
+// from that buffer at once; if move k is accepted, the buffer is dropped
+// (nothing was consumed from the stream) and move k+1 is drawn again after
+// the commit.  The chain is the one k_mc_chunk produces.  No read-ahead is
+// done across a step after which dmax is adapted.
+
+// The draw stream with the batch after the current one already generated,
+// so that up to 32 draws beyond the current batch can be read ahead.
+struct LookDraws {
+    WaveDraws d;   // current batch (d.v, d.cur) and the generator
+    double v1;     // the next batch
+
+    __device__ __forceinline__ void gen(int lane)
+    {
+        const double keep = d.v;
+        const int c = d.cur;
+        d.refill(lane);
+        v1 = d.v;
+        d.v = keep;
+        d.cur = c;
+    }
+    __device__ __forceinline__ void load(const uint32_t *st, int lane)
+    {
+        d.load(st, lane);
+        if (st[kRunRngWords - 1])
+            v1 = lane < 32 ? __hiloint2double((int)st[kRngWords + 2 * lane + 1], (int)st[kRngWords + 2 * lane]) : 0.0;
+        else
+            gen(lane);
+    }
+    __device__ __forceinline__ void save(uint32_t *st, int lane) const
+    {
+        d.save(st, lane);
+        if (lane < 32) {
+            st[kRngWords + 2 * lane] = (uint32_t)__double2loint(v1);
+            st[kRngWords + 2 * lane + 1] = (uint32_t)__double2hiint(v1);
+        }
+        if (lane == 0) st[kRunRngWords - 1] = 1u;
+    }
+    __device__ __forceinline__ double next(int lane)
+    {
+        if (d.cur == 32) {
+            d.v = v1;
+            d.cur = 0;
+            gen(lane);
+        }
+        const double u = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(d.v), d.cur),
+                                          __builtin_amdgcn_readlane(__double2loint(d.v), d.cur));
+        d.cur++;
+        return u;
+    }
+};
+
+// Read-ahead cursor over the two batches of a LookDraws (nothing consumed).
+// Past their end it returns 0.5 (which ends the unit-ball rejection loop)
+// and clears ok: that proposal is dropped.
+struct PeekDraws {
+    double v0, v1;
+    int idx;       // draw index from the start of the current batch
+    bool ok;
+    __device__ __forceinline__ double next(int)
+    {
+        double u = 0.5;
+        if (idx < 64) {
+            const double src = idx < 32 ? v0 : v1;
+            const int l = idx & 31;
+            u = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(src), l),
+                                 __builtin_amdgcn_readlane(__double2loint(src), l));
+        } else {
+            ok = false;
+        }
+        idx++;
+        return u;
+    }
+};
+
+// Is dmax adapted after (1-based) step st?  (src/MC_metropolis.cpp:137-148)
+__device__ __forceinline__ bool adapts_at(const DevMoves &mv, long long st)
+{
+    for (int k = 0; k < mv.ntypes; k++)
+        if (mv.each[k] != 0 && (st % mv.each[k]) == 0) return true;
+    return false;
+}
+
+struct PipeLDS {
+    MoveLDS m[2];          // the move under evaluation and the read-ahead one
+    int n;                 // 1: evaluate m[cur]; 0: the chunk is over
+    int cur;
+    int stop;
+};
+
+// The Metropolis loop with read-ahead proposals, for move sets without
+// force-bias moves; same arguments and the same chain as k_mc_chunk, and
+// the same three workgroup barriers per step in both role loops.
+template <int PB, int CUT>
+__global__ __launch_bounds__(kMCT) void k_mc_spec(DevSys s0, DevTerms t, DevMoves mv, MCRunArgs arg)
+{
+    const DevSys s = specialise<PB, CUT>(s0);
+    if (arg.bail && (s.overflow[0] != 0 || arg.bail[0] != INT_MAX)) {
+        // the list this chunk would use is incomplete: leave every state
+        // untouched; the host grows the list and runs again from here
+        if (threadIdx.x == 0) atomicMin(arg.bail, arg.step0);
+        return;
+    }
+    __shared__ PipeLDS g;
+    __shared__ uint32_t mt[kMTN];
+    __shared__ double dmax[kMaxMoveTypes];
+    __shared__ int acc_tmp[kMaxMoveTypes];
+    __shared__ long long n_trial[kMaxMoveTypes], n_acc[kMaxMoveTypes];
+    __shared__ double etot;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const bool w0 = __builtin_amdgcn_readfirstlane(tid) < 64;   // wave-uniform
+    if (arg.rng_mode == 1)
+        for (int k = tid; k < kMTN; k += kMCT) mt[k] = arg.mt_state[k];
+    if (tid == 0) g.stop = 0;
+    __syncthreads();
+    LookDraws ld;
+    if (w0) {
+        ld.d.replay = arg.rng_mode == 1;
+        ld.d.mt = mt;
+        ld.d.key = make_uint2((uint32_t)arg.seed, (uint32_t)(arg.seed >> 32));
+        ld.d.chain = arg.stream;
+        ld.load(arg.mt_state, lane);
+        if (lane < kMaxMoveTypes && lane < mv.ntypes) {
+            dmax[lane] = arg.dmax[lane]; acc_tmp[lane] = arg.acc_tmp[lane];
+            n_trial[lane] = arg.trials[lane]; n_acc[lane] = arg.accepted[lane];
+        }
+        if (lane == 0) etot = arg.etot[0];
+        wave_sync();
+    }
+
+    int step = 0;
+    if (w0) {
+        bool stop = false, ready = false;
+        int cur = 0;
+        for (;;) {
+            int n = 0;
+            if (step < arg.nsteps && !stop) {
+                if (!ready) pick_load(s, mv, ld, dmax, g.m[cur], lane, arg.ipos_ref, arg.half_skin2);
+                n = 1;
+            }
+            if (lane == 0) { g.n = n; g.cur = cur; }
+            __syncthreads();                  // the move is published
+            if (n == 0) break;
+            MoveLDS &m = g.m[cur];
+            double dec = 0.0, dev = 0.0, deb = 0.0;
+            double fnew[3] = {0.0, 0.0, 0.0};
+            delta_terms(s, mv, m, dec, dev, deb);
+            // read ahead: the next proposal, valid if this move is rejected
+            // (then its acceptance draw was taken)
+            int nd = -1;
+            if (step + 1 < arg.nsteps && !adapts_at(mv, (long long)arg.step0 + step + 1)) {
+                PeekDraws pk{ld.d.v, ld.v1, ld.d.cur + 1, true};
+                const int i0 = pk.idx;
+                pick_load(s, mv, pk, dmax, g.m[cur ^ 1], lane, arg.ipos_ref, arg.half_skin2);
+                if (pk.ok) nd = pk.idx - i0;
+            }
+            delta_finish(m, false, dec, dev, deb, fnew);
+            // Metropolis test (apply_criterion, src/MC_metropolis.cpp:175-204),
+            // bookkeeping, commit of the moved atoms
+            const int st = arg.step0 + step + 1;
+            const int tp = m.acc;
+            const double de = m.de;
+            const int acc = metropolis(ld, arg.beta, de, lane);
+            if (lane == 0) {
+                n_trial[tp]++;
+                if (acc) { etot += de; n_acc[tp]++; }
+                const int o = st - 1;
+                if (arg.acc_out) arg.acc_out[o] = (unsigned char)acc;
+                if (arg.de_out) arg.de_out[o] = de;
+                if (arg.mvtyp_out) arg.mvtyp_out[o] = tp;
+                if (arg.mvinst_out) arg.mvinst_out[o] = m.inst;
+                if (arg.lnq_out) arg.lnq_out[o] = 0.0;
+                if (arg.disp_out) {
+                    const bool tr = m.type != MCX_MV_ROT_DEV;
+                    for (int c = 0; c < 3; c++) arg.disp_out[3 * o + c] = tr ? m.r[c] : 0.0;
+                }
+                adapt_dmax(mv, st, tp, acc, dmax, acc_tmp);
+            }
+            step++;
+            if (acc) {
+                bool far = false;
+                if (lane < m.nm) {
+                    const int i = m.first + lane;
+                    s.x[i] = m.tx[lane]; s.y[i] = m.ty[lane]; s.z[i] = m.tz[lane];
+                    const int4 np = mcx_fixed4(s, m.tx[lane], m.ty[lane], m.tz[lane], __int_as_float(m.ip[lane].w));
+                    s.ipos[i] = np;
+                    far = arg.ipos_ref && beyond_half_skin(s, np, arg.ipos_ref[i], arg.half_skin2);
+                }
+                if (__ballot(far) != 0ull) stop = true;
+                ready = false;
+            } else if (nd >= 0) {
+                // the read-ahead proposal is the next move: take its draws
+                for (int q = 0; q < nd; q++) (void)ld.next(lane);
+                cur ^= 1;
+                ready = true;
+            } else {
+                ready = false;
+            }
+            wave_sync();
+        }
+        if (lane == 0) g.stop = stop;
+    } else {
+        for (;;) {
+            __syncthreads();                  // the move is published
+            if (g.n == 0) break;
+            MoveLDS &m = g.m[g.cur];
+            double dec = 0.0, dev = 0.0;
+            double fnew[3] = {0.0, 0.0, 0.0};
+            delta_pairs(s, m, false, dec, dev, fnew);
+            delta_finish(m, false, dec, dev, 0.0, fnew);
+        }
+    }
+    if (arg.stop_at && tid == 0) {
+        arg.stop_at[0] = step;
+        arg.stop_at[1] = g.stop;
+    }
+    __syncthreads();
+    if (w0) {
+        if (lane < mv.ntypes) {
+            arg.dmax[lane] = dmax[lane]; arg.acc_tmp[lane] = acc_tmp[lane];
+            arg.trials[lane] = n_trial[lane]; arg.accepted[lane] = n_acc[lane];
+        }
+        if (lane == 0) arg.etot[0] = etot;
+        ld.save(arg.mt_state, lane);
+    }
+    if (arg.rng_mode == 1)
+        for (int k = tid; k < kMTN; k += kMCT) arg.mt_state[k] = mt[k];
+}
+
 // one-shot dE for caller-supplied trial coordinates
 __global__ __launch_bounds__(kMCT) void k_delta(DevSys s, DevTerms t, DevMoves mv, int first, int last,
                                                 const double *__restrict__ nxyz, double *out)
@@ -673,7 +913,14 @@ void launch_mc_chunk(const DevSys &s, const DevTerms &t, const DevMoves &mv, con
 {
     // the water and argon boxes (cubic, switched) and cubic all-pair runs get
     // their own instances; other systems run the generic one
-    if (s.pbtype == MCX_PBC_CUBIC && s.cutmode == MCX_CUT_SWITCH)
+    if (a.spec) {
+        if (s.pbtype == MCX_PBC_CUBIC && s.cutmode == MCX_CUT_SWITCH)
+            hipLaunchKernelGGL((k_mc_spec<MCX_PBC_CUBIC, MCX_CUT_SWITCH>), dim3(1), dim3(kMCT), 0, st, s, t, mv, a);
+        else if (s.pbtype == MCX_PBC_CUBIC && s.cutmode == MCX_CUT_FULL)
+            hipLaunchKernelGGL((k_mc_spec<MCX_PBC_CUBIC, MCX_CUT_FULL>), dim3(1), dim3(kMCT), 0, st, s, t, mv, a);
+        else
+            hipLaunchKernelGGL((k_mc_spec<-1, -1>), dim3(1), dim3(kMCT), 0, st, s, t, mv, a);
+    } else if (s.pbtype == MCX_PBC_CUBIC && s.cutmode == MCX_CUT_SWITCH)
         hipLaunchKernelGGL((k_mc_chunk<MCX_PBC_CUBIC, MCX_CUT_SWITCH>), dim3(1), dim3(kMCT), 0, st, s, t, mv, a);
     else if (s.pbtype == MCX_PBC_CUBIC && s.cutmode == MCX_CUT_FULL)
         hipLaunchKernelGGL((k_mc_chunk<MCX_PBC_CUBIC, MCX_CUT_FULL>), dim3(1), dim3(kMCT), 0, st, s, t, mv, a);

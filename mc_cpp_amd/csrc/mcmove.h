@@ -234,7 +234,9 @@ __device__ __forceinline__ DevSys specialise(const DevSys &s0)
 // direction (rejection in the unit ball) and, for rotations, the angle and
 // the rotation matrix.  A force-bias move draws a Gaussian vector instead
 // (Box-Muller; left in r for the drift to be added).
-__device__ __forceinline__ void pick_instance(const DevMoves &mv, WaveDraws &dr, int lane, int &tp, int &in,
+// The draw source D is anything with a wave-uniform next(lane).
+template <class D>
+__device__ __forceinline__ void pick_instance(const DevMoves &mv, D &dr, int lane, int &tp, int &in,
                                               int4 &rec, int2 &rr)
 {
     tp = (int)((double)mv.ntypes * dr.next(lane));
@@ -243,7 +245,8 @@ __device__ __forceinline__ void pick_instance(const DevMoves &mv, WaveDraws &dr,
     rr = mv.inst_adj[mv.off[tp] + in];
 }
 
-__device__ __forceinline__ void pick_displacement(WaveDraws &dr, const double *dmax, int tp, int type, int lane,
+template <class D>
+__device__ __forceinline__ void pick_displacement(D &dr, const double *dmax, int tp, int type, int lane,
                                                   double (&r)[3], double (&R)[3][3])
 {
     if (type == MCX_MV_FBIAS_DEV) {
@@ -293,7 +296,8 @@ __device__ __forceinline__ void pick_move(const DevMoves &mv, WaveDraws &dr, con
 
 // MC_metropolis::apply_criterion (src/MC_metropolis.cpp:175-204): a draw is
 // consumed only when dE exceeds DBL_EPSILON.
-__device__ __forceinline__ int metropolis(WaveDraws &dr, double beta, double de, int lane)
+template <class D>
+__device__ __forceinline__ int metropolis(D &dr, double beta, double de, int lane)
 {
     if (de <= DBL_EPSILON) return 1;
     const double alpha = dr.next(lane);

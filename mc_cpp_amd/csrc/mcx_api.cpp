@@ -7,6 +7,7 @@
 #include <chrono>
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -924,7 +925,7 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
         std::vector<double> dm(ntypes);
         for (int t = 0; t < ntypes; t++) dm[t] = types[t].dmax;
         dmax.upload(dm.data(), ntypes, h->stream);
-        std::vector<uint32_t> st(692, 0);
+        std::vector<uint32_t> st(kRunRngWords, 0);
         mt_seed(st.data(), seed);
         st[625] = 0; st[626] = 0;
         st[627] = 32;     // empty draw batch: the first draw refills it
@@ -945,6 +946,12 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
         a.seed = seed;
         a.stream = h->rng_stream;
         a.beta = 1.0 / ((MCX_RBOLTZUI / MCX_KCALTOIU) * h->temp);
+        // speculative rounds (several proposals evaluated together; the same
+        // chain) unless a force-bias type is present or MCX_SPECULATE=0
+        const char *sp = std::getenv("MCX_SPECULATE");
+        a.spec = !(sp && sp[0] == '0');
+        for (int k = 0; k < ntypes; k++)
+            if (types[k].type == MCX_MV_FBIAS) a.spec = 0;
         a.etot = etot.p; a.mt_state = mts.p; a.dmax = dmax.p; a.trials = trials.p; a.accepted = accepted.p;
         a.acc_tmp = acc_tmp.p;
         a.acc_out = acc_out ? acc_d.p : nullptr;

@@ -38,6 +38,9 @@ constexpr int kMaxMoveTypes = 8;
 // index, 625/626 Philox block counter (hi, lo), 627 cursor into the draw
 // batch, [628,692) the batch (32 doubles).
 constexpr int kRngWords = 692;
+// The single-chain state adds the batch after the current one, drawn ahead
+// for speculative proposals: [692,756) (32 doubles) and 756 (1: present).
+constexpr int kRunRngWords = 757;
 
 // Everything a kernel needs about one system, passed by value.
 // Per-atom parameters of the all-pair kernel's j atom, read as one 16-byte
@@ -196,6 +199,8 @@ struct MCRunArgs {
     // chunk that finds the preceding rebuild overflowed (s.overflow) or an
     // earlier chunk bailed out changes nothing and records its first step
     int *bail;             // device scalar (INT_MAX: none), or null
+    // evaluate several proposals per round (no force-bias move type)
+    int spec;
 };
 void launch_mc_chunk(const DevSys &s, const DevTerms &t, const DevMoves &mv, const MCRunArgs &a,
                      hipStream_t st);

@@ -259,6 +259,42 @@ def test_mc_replay_small_box():
     assert abs(ff.getE() - Oracle(ref).getE()[0]) <= 1e-6 * max(e0, abs(ro["etot"]))
 
 
+@pytest.mark.parametrize("case", ["water_every50_adapt", "water_auto_philox", "argon_philox_adapt"])
+def test_mc_read_ahead_same_chain(case, monkeypatch):
+    """Move sets without force-bias moves run the read-ahead kernel (the next
+    proposal drawn and loaded while the current move's pairs are evaluated);
+    MCX_SPECULATE=0 selects the one-proposal-at-a-time kernel.  Both give the
+    same chain bit for bit: the same dE, decisions, dmax and coordinates,
+    through list rebuilds, chunk boundaries (the draw state carried over)
+    and dmax adaptation steps (where no read-ahead is done)."""
+    import copy
+    if case.startswith("water"):
+        s, _ = systems.water_box(seed=3, nside=8, L=24.85)
+    else:
+        s, _ = systems.argon_fcc(seed=1)
+    for k, m in enumerate(s.moves):
+        m.each = 70 + 40 * k if case.endswith("adapt") else 0
+    mode = mcx.MC_REPLAY if "philox" not in case else mcx.MC_PHILOX
+    every = mcx.LIST_AUTO if "auto" in case else 50
+    n = 4000
+    out = {}
+    for spec in ("0", "1"):
+        monkeypatch.setenv("MCX_SPECULATE", spec)
+        sc = copy.deepcopy(s)
+        ff = mcx.FField(sc)
+        res = ff.h.mc_run(sc.moves, n, 97, rng_mode=mode, list_every=every)
+        out[spec] = (res, ff.get_coords())
+    (r0, c0), (r1, c1) = out["0"], out["1"]
+    assert 0 < int(r0["acc"].sum()) < n
+    for key in ("acc", "de", "mvtyp", "mvinst", "disp", "trials", "accepted", "dmax"):
+        assert np.array_equal(r0[key], r1[key]), key
+    assert r0["etot"] == r1["etot"]
+    for a, b in zip(c0, c1):
+        assert np.array_equal(a, b)
+    if case.endswith("adapt"):
+        assert not np.array_equal(r0["dmax"], np.array([m.dmax for m in s.moves]))
+
+
 def _replay_system(kind):
     if kind == "cubic_full":
         s, _ = systems.water_box(seed=5, nside=5, L=15.53, nmol=125, tag=41)
