@@ -166,30 +166,41 @@ __device__ __forceinline__ void delta_pairs(const DevSys &s, const MoveLDS &m, b
     const int pt = threadIdx.x - 64;
     const int total = m.total, nm = m.nm;
     const bool all = s.cutmode == 0 || m.scan;
+    // Row starts of a set of up to four atoms, read once: the moved atom of
+    // an entry is then found without a chain of dependent LDS reads.
+    const bool few = nm <= 4;
+    const int p1 = nm > 1 ? m.pref[1] : INT_MAX, p2 = nm > 2 ? m.pref[2] : INT_MAX;
+    const int p3 = nm > 3 ? m.pref[3] : INT_MAX;
     for (int base = 0; base < total; base += kPairU * NP) {
+        // entries past the end are clamped to the last one (and dropped
+        // below), so that every load is unconditional and all of them are
+        // in flight together
         int av[kPairU], jv[kPairU];
 #pragma unroll
         for (int u = 0; u < kPairU; u++) {
-            const int e = base + u * NP + pt;
-            av[u] = -1;
-            jv[u] = -1;
-            if (e < total) {
-                int lo = 0, hi = nm - 1;       // last a with pref[a] <= e
+            const int e0 = base + u * NP + pt;
+            const int e = e0 < total ? e0 : total - 1;
+            int lo, start;
+            if (few) {
+                lo = (e >= p1) + (e >= p2) + (e >= p3);
+                start = lo == 0 ? 0 : lo == 1 ? p1 : lo == 2 ? p2 : p3;
+            } else {
+                lo = 0;                        // last a with pref[a] <= e
+                int hi = nm - 1;
                 while (lo < hi) {
                     const int mid = (lo + hi + 1) >> 1;
                     if (m.pref[mid] <= e) lo = mid; else hi = mid - 1;
                 }
-                const int k = e - m.pref[lo];
-                av[u] = lo;
-                jv[u] = all ? k : s.nbr[(size_t)(m.first + lo) * s.cap + k];
+                start = m.pref[lo];
             }
+            const int k = e - start;
+            av[u] = e0 < total ? lo : -1;
+            jv[u] = all ? k : s.nbr[(size_t)(m.first + lo) * s.cap + k];
         }
         int4 pj[kPairU];
         double4 pq[kPairU];
 #pragma unroll
-        for (int u = 0; u < kPairU; u++) {
-            if (jv[u] >= 0) { pj[u] = s.ipos[jv[u]]; pq[u] = s.qes[jv[u]]; }
-        }
+        for (int u = 0; u < kPairU; u++) { pj[u] = s.ipos[jv[u]]; pq[u] = s.qes[jv[u]]; }
 #pragma unroll
         for (int u = 0; u < kPairU; u++) {
             const int a = av[u], j = jv[u];
