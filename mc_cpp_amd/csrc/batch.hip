@@ -31,21 +31,33 @@ namespace {
 constexpr int kQ = 128;     // entries of a wavefront's candidate queue
 constexpr int kBRec = 32;   // term records of a move staged in LDS
 
-// The move as wavefront 0 publishes it for the scan.
+// The move as wavefront 0 publishes it for the scan and commits it: two
+// copies, the move under evaluation and the one read ahead.
 struct BMove {
     int first, last, nm, check;
     float thr;                 // squared radius of the candidate filter (A^2)
     int4 ip[kBatchAtoms];      // old fixed-point positions, type word (index | frozen << 16) in .w
     double gx[kBatchAtoms], gy[kBatchAtoms], gz[kBatchAtoms];   // unwrapped displacements
     double tx[kBatchAtoms], ty[kBatchAtoms], tz[kBatchAtoms];   // trial coordinates (wrapped)
+    double4 qe[kBatchAtoms];   // (charge, eps, sig, 0)
+};
+
+// What only wavefront 0 reads before it reads the next move ahead (one
+// copy), and the per-wavefront partial sums.
+struct BStage {
     double ox[kBatchAtoms], oy[kBatchAtoms], oz[kBatchAtoms];   // old coordinates
     double ms[kBatchAtoms];
-    double4 qe[kBatchAtoms];   // (charge, eps, sig, 0)
     double red[16];
     int4 rid[kBRec];           // the move's first kBRec term records
     int rkk[kBRec], rty[kBRec];
     double4 rpar[kBRec];
     int2 pent[kBRec];          // and pair corrections
+};
+
+// Wavefront 0's registers of a move.
+struct BRegs {
+    int tp, first, last, nm, ra0, ra1, pa0, pa1;
+    bool rigid;
 };
 
 // Per-type move parameters of a replica (the DevMoves fields indexed by
@@ -214,13 +226,151 @@ __device__ __forceinline__ double scan_pairs(const DevSys &s, const BMove &m, co
     return MCX_CHGCHARMM * MCX_KCALTOIU * dec + 4.0 * dev;
 }
 
+// Phase A of a move (wavefront 0, lanes in step): the move
+// (src/MC_metropolis.cpp:85-116; src/MC.cpp:137-156), the moved atoms and
+// the trial coordinates (MOVE_TRN::tr_a_b / MOVE_ROT::apply_rotation,
+// include/Move.hpp:42-50,120-143), with no barrier in between, into m and
+// sg; the draws come from D (the stream, or a read-ahead cursor).
+template <bool IMG, class D>
+__device__ __forceinline__ BRegs batch_pick(const DevSys &s, const DevMoves &mv, const MoveTab &tab,
+                                            const double *dmax, const double4 *ptab, const ImgFetch<IMG> &f,
+                                            D &dr, BMove &m, BStage &sg, int lane)
+{
+    int tp = 0, first = 0, last = 0, nm = 0, ra0 = 0, ra1 = 0, pa0 = 0, pa1 = 0, inst = 0;
+    bool rigid = false;
+    tp = (int)((double)tab.ntypes * dr.next(lane));
+    inst = (int)((double)tab.ninst[tp] * dr.next(lane));
+    const int k = tab.off[tp] + inst;
+    const int4 rec = mv.inst[k];
+    const int2 rr = mv.inst_adj[k];
+    const int2 rp = mv.inst_pairs[k];
+    double rv[3];
+    do {
+        rv[0] = dr.next(lane) * 2.0 + (-1.0);
+        rv[1] = dr.next(lane) * 2.0 + (-1.0);
+        rv[2] = dr.next(lane) * 2.0 + (-1.0);
+    } while ((rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]) > 1.0);
+    const int type = tab.type[tp];
+    double Rm[3][3];
+    if (type == MCX_MV_TRN_DEV) {
+        rv[0] = rv[0] * dmax[tp]; rv[1] = rv[1] * dmax[tp]; rv[2] = rv[2] * dmax[tp];
+    } else {
+        const double rang = dmax[tp] * (dr.next(lane) * 2.0 + (-1.0));
+        rot_mat(Rm, rv, rang);
+    }
+    first = rec.x; last = rec.y;
+    const int pivot = rec.z;
+    nm = last - first + 1;
+    ra0 = rr.x; ra1 = rr.y;
+    pa0 = rp.x; pa1 = rp.y;
+    double x = 0.0, y = 0.0, z = 0.0;
+    int4 pl = make_int4(0, 0, 0, 0);
+    if (lane < nm) {
+        const int i = first + lane;
+        x = s.x[i]; y = s.y[i]; z = s.z[i];
+        pl = f(i);
+        sg.ox[lane] = x; sg.oy[lane] = y; sg.oz[lane] = z;
+        sg.ms[lane] = (pivot < 0 && mv.mass) ? mv.mass[i] : 1.0;
+    }
+    if (lane < kBRec && lane < ra1 - ra0) {
+        const int e = ra0 + lane;
+        sg.rid[lane] = mv.rec_ids[e]; sg.rkk[lane] = mv.rec_kk[e]; sg.rty[lane] = mv.rec_type[e];
+        sg.rpar[lane] = mv.rec_par[e];
+    }
+    if (lane < kBRec && lane < pa1 - pa0) sg.pent[lane] = mv.pair_ent[pa0 + lane];
+    double piv[3] = {0.0, 0.0, 0.0};
+    if (type != MCX_MV_TRN_DEV) {
+        if (pivot >= 0 && (pivot < first || pivot > last)) {
+            // a uniform address this wavefront writes with vector
+            // stores: atomic loads are never compiled as scalar-cache
+            // loads, which would not see those stores
+            piv[0] = __hip_atomic_load(s.x + pivot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            piv[1] = __hip_atomic_load(s.y + pivot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            piv[2] = __hip_atomic_load(s.z + pivot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+            wave_sync();
+            if (pivot >= 0) {
+                piv[0] = sg.ox[pivot - first]; piv[1] = sg.oy[pivot - first]; piv[2] = sg.oz[pivot - first];
+            } else {   // Tools::getCentreOfMass, include/Tools.hpp:95-113
+                double mt_ = 0.0, c0 = 0.0, c1 = 0.0, c2 = 0.0;
+                for (int b = 0; b < nm; b++) {
+                    const double ms = sg.ms[b];
+                    mt_ += ms; c0 += ms * sg.ox[b]; c1 += ms * sg.oy[b]; c2 += ms * sg.oz[b];
+                }
+                piv[0] = c0 / mt_; piv[1] = c1 / mt_; piv[2] = c2 / mt_;
+            }
+        }
+    }
+    const int p0x = __builtin_amdgcn_readlane(pl.x, 0), p0y = __builtin_amdgcn_readlane(pl.y, 0);
+    const int p0z = __builtin_amdgcn_readlane(pl.z, 0);
+    double g2 = 0.0, rad2 = 0.0;
+    bool far = false;
+    if (lane < nm) {
+        double ux, uy, uz;
+        if (type == MCX_MV_TRN_DEV) {
+            ux = x + rv[0]; uy = y + rv[1]; uz = z + rv[2];
+        } else {
+            const double px = x - piv[0], py = y - piv[1], pz = z - piv[2];
+            const double lq = 0.25 * fmin(s.L[0], fmin(s.L[1], s.L[2]));
+            far = s.pbtype != 0 && px * px + py * py + pz * pz >= lq * lq;
+            ux = piv[0] + Rm[0][0] * px + Rm[1][0] * py + Rm[2][0] * pz;
+            uy = piv[1] + Rm[0][1] * px + Rm[1][1] * py + Rm[2][1] * pz;
+            uz = piv[2] + Rm[0][2] * px + Rm[1][2] * py + Rm[2][2] * pz;
+        }
+        const double ddx = ux - x, ddy = uy - y, ddz = uz - z;
+        m.gx[lane] = ddx; m.gy[lane] = ddy; m.gz[lane] = ddz;
+        m.tx[lane] = wrap1(s, ux, 0); m.ty[lane] = wrap1(s, uy, 1); m.tz[lane] = wrap1(s, uz, 2);
+        m.ip[lane] = pl;
+        m.qe[lane] = ptab[pl.w & 0xffff];
+        g2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        const double rx = (double)mcx_wdiff(pl.x, p0x) * s.unit[0];
+        const double ry = (double)mcx_wdiff(pl.y, p0y) * s.unit[1];
+        const double rz = (double)mcx_wdiff(pl.z, p0z) * s.unit[2];
+        rad2 = rx * rx + ry * ry + rz * rz;
+    }
+    const double gm = sqrt(wave_max0(g2)), rm = sqrt(wave_max0(rad2));
+    // A translation, or a rotation whose atoms all lie within a
+    // quarter box of the pivot, moves the set rigidly: distances,
+    // angles and torsions inside it change by rounding only
+    // (~1e-15 relative), so the terms among its own atoms are
+    // skipped.  A rotation of a set the box edge splits (the pivot
+    // is applied to wrapped coordinates, include/Move.hpp:120-143)
+    // is not rigid and keeps them.
+    rigid = __ballot(far) == 0;
+    if (lane == 0) {
+        // |x_j - x_a| >= |x_j - x_0| - rm and a moves by at most gm, so
+        // a candidate farther than rc + rm + gm from atom 0's old
+        // position is outside the cutoff before and after the move;
+        // inside that sphere, o + g stays within the half box when
+        // rc + rm + 2 gm < L/2 on every axis
+        const double rc = sqrt(s.ctoff2) + rm + gm;
+        const double reach = rc + gm;
+        m.first = first; m.last = last; m.nm = nm;
+        m.thr = s.cutmode == 0 ? __builtin_huge_valf() : (float)(rc * rc * 1.00001 + 1e-4);
+        m.check = s.cutmode == 0 ||
+                  !(reach * 1.00001 < 0.5 * s.L[0] && reach * 1.00001 < 0.5 * s.L[1] &&
+                    reach * 1.00001 < 0.5 * s.L[2]);
+    }
+    return BRegs{tp, first, last, nm, ra0, ra1, pa0, pa1, rigid};
+}
+
+// Is dmax adapted after (1-based) step st?  (src/MC.cpp:165-173)
+__device__ __forceinline__ bool tab_adapts(const MoveTab &tab, long long st)
+{
+    for (int k = 0; k < tab.ntypes; k++)
+        if (tab.each[k] != 0 && (st % tab.each[k]) == 0) return true;
+    return false;
+}
+
 template <int NT, int U, bool IMG, bool CS>
 __global__ __launch_bounds__(NT) void k_batch_sweep(BatchArgs a)
 {
     using QT = typename std::conditional<IMG, unsigned short, int>::type;
     constexpr int NW = NT / 64;
     extern __shared__ int4 dyn[];
-    __shared__ BMove m;
+    __shared__ BMove mb[2];
+    __shared__ BStage sg;
+    __shared__ int mcur;
     __shared__ MoveTab tab;
     __shared__ uint32_t mt[kMTN];
     __shared__ double4 ptab[kBatchTypes];
@@ -249,15 +399,14 @@ __global__ __launch_bounds__(NT) void k_batch_sweep(BatchArgs a)
         }
     for (int k = tid; k < R.ntab; k += NT) ptab[k] = R.ptab[k];
     const double beta = 1.0 / ((MCX_RBOLTZUI / MCX_KCALTOIU) * a.temps[a.slot_of_rep[r]]);
-    WaveDraws dr;
+    LookDraws ld;
     double etot = 0.0;
     long long first_step = 0;
     if (w0) {
-        dr.replay = R.rng_mode == 1;
-        dr.mt = mt;
-        dr.key = make_uint2((uint32_t)R.key, (uint32_t)(R.key >> 32));
-        dr.chain = R.stream;
-        dr.load(R.rng, lane);
+        ld.d.replay = R.rng_mode == 1;
+        ld.d.mt = mt;
+        ld.d.key = make_uint2((uint32_t)R.key, (uint32_t)(R.key >> 32));
+        ld.d.chain = R.stream;
         const int nt = mv.ntypes;
         if (lane < nt) {
             dmax[lane] = R.dmax[lane]; acc_tmp[lane] = R.acc_tmp[lane];
@@ -272,6 +421,7 @@ __global__ __launch_bounds__(NT) void k_batch_sweep(BatchArgs a)
     if (R.rng_mode == 1)
         for (int k = tid; k < kMTN; k += NT) mt[k] = R.rng[k];
     __syncthreads();
+    if (w0) ld.load(R.rng, lane);      // may generate the read-ahead batch from mt
     const int nmoves = a.nmoves > 0 ? a.nmoves : R.sweep_len;
     unsigned npair = 0;     // this lane's in-cutoff pair evaluations (old and new) of the scan
 
@@ -280,134 +430,25 @@ __global__ __launch_bounds__(NT) void k_batch_sweep(BatchArgs a)
     // the other wavefronts the pair scan, so that each role's values stay
     // out of the other's register allocation.
     if (w0) {
+        int c = 0;                 // buffer of the move under evaluation
+        bool ready = false;        // mb[c] already holds it (read ahead)
+        BRegs q{};
         for (int mi = 0; mi < nmoves; mi++) {
-            // ---- wavefront 0: the move (src/MC_metropolis.cpp:85-116;
-            // src/MC.cpp:137-156), the moved atoms and the trial coordinates
-            // (MOVE_TRN::tr_a_b / MOVE_ROT::apply_rotation,
-            // include/Move.hpp:42-50,120-143), with no barrier in between
-            int tp = 0, first = 0, last = 0, nm = 0, ra0 = 0, ra1 = 0, pa0 = 0, pa1 = 0, inst = 0;
-            bool rigid = false;
-            tp = (int)((double)tab.ntypes * dr.next(lane));
-            inst = (int)((double)tab.ninst[tp] * dr.next(lane));
-            const int k = tab.off[tp] + inst;
-            const int4 rec = mv.inst[k];
-            const int2 rr = mv.inst_adj[k];
-            const int2 rp = mv.inst_pairs[k];
-            double rv[3];
-            do {
-                rv[0] = dr.next(lane) * 2.0 + (-1.0);
-                rv[1] = dr.next(lane) * 2.0 + (-1.0);
-                rv[2] = dr.next(lane) * 2.0 + (-1.0);
-            } while ((rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]) > 1.0);
-            const int type = tab.type[tp];
-            double Rm[3][3];
-            if (type == MCX_MV_TRN_DEV) {
-                rv[0] = rv[0] * dmax[tp]; rv[1] = rv[1] * dmax[tp]; rv[2] = rv[2] * dmax[tp];
-            } else {
-                const double rang = dmax[tp] * (dr.next(lane) * 2.0 + (-1.0));
-                rot_mat(Rm, rv, rang);
-            }
-            first = rec.x; last = rec.y;
-            const int pivot = rec.z;
-            nm = last - first + 1;
-            ra0 = rr.x; ra1 = rr.y;
-            pa0 = rp.x; pa1 = rp.y;
-            double x = 0.0, y = 0.0, z = 0.0;
-            int4 pl = make_int4(0, 0, 0, 0);
-            if (lane < nm) {
-                const int i = first + lane;
-                x = s.x[i]; y = s.y[i]; z = s.z[i];
-                pl = f(i);
-                m.ox[lane] = x; m.oy[lane] = y; m.oz[lane] = z;
-                m.ms[lane] = (pivot < 0 && mv.mass) ? mv.mass[i] : 1.0;
-            }
-            if (lane < kBRec && lane < ra1 - ra0) {
-                const int e = ra0 + lane;
-                m.rid[lane] = mv.rec_ids[e]; m.rkk[lane] = mv.rec_kk[e]; m.rty[lane] = mv.rec_type[e];
-                m.rpar[lane] = mv.rec_par[e];
-            }
-            if (lane < kBRec && lane < pa1 - pa0) m.pent[lane] = mv.pair_ent[pa0 + lane];
-            double piv[3] = {0.0, 0.0, 0.0};
-            if (type != MCX_MV_TRN_DEV) {
-                if (pivot >= 0 && (pivot < first || pivot > last)) {
-                    // a uniform address this wavefront writes with vector
-                    // stores: atomic loads are never compiled as scalar-cache
-                    // loads, which would not see those stores
-                    piv[0] = __hip_atomic_load(s.x + pivot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    piv[1] = __hip_atomic_load(s.y + pivot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    piv[2] = __hip_atomic_load(s.z + pivot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                } else {
-                    wave_sync();
-                    if (pivot >= 0) {
-                        piv[0] = m.ox[pivot - first]; piv[1] = m.oy[pivot - first]; piv[2] = m.oz[pivot - first];
-                    } else {   // Tools::getCentreOfMass, include/Tools.hpp:95-113
-                        double mt_ = 0.0, c0 = 0.0, c1 = 0.0, c2 = 0.0;
-                        for (int b = 0; b < nm; b++) {
-                            const double ms = m.ms[b];
-                            mt_ += ms; c0 += ms * m.ox[b]; c1 += ms * m.oy[b]; c2 += ms * m.oz[b];
-                        }
-                        piv[0] = c0 / mt_; piv[1] = c1 / mt_; piv[2] = c2 / mt_;
-                    }
-                }
-            }
-            const int p0x = __builtin_amdgcn_readlane(pl.x, 0), p0y = __builtin_amdgcn_readlane(pl.y, 0);
-            const int p0z = __builtin_amdgcn_readlane(pl.z, 0);
-            double g2 = 0.0, rad2 = 0.0;
-            bool far = false;
-            if (lane < nm) {
-                double ux, uy, uz;
-                if (type == MCX_MV_TRN_DEV) {
-                    ux = x + rv[0]; uy = y + rv[1]; uz = z + rv[2];
-                } else {
-                    const double px = x - piv[0], py = y - piv[1], pz = z - piv[2];
-                    const double lq = 0.25 * fmin(s.L[0], fmin(s.L[1], s.L[2]));
-                    far = s.pbtype != 0 && px * px + py * py + pz * pz >= lq * lq;
-                    ux = piv[0] + Rm[0][0] * px + Rm[1][0] * py + Rm[2][0] * pz;
-                    uy = piv[1] + Rm[0][1] * px + Rm[1][1] * py + Rm[2][1] * pz;
-                    uz = piv[2] + Rm[0][2] * px + Rm[1][2] * py + Rm[2][2] * pz;
-                }
-                const double ddx = ux - x, ddy = uy - y, ddz = uz - z;
-                m.gx[lane] = ddx; m.gy[lane] = ddy; m.gz[lane] = ddz;
-                m.tx[lane] = wrap1(s, ux, 0); m.ty[lane] = wrap1(s, uy, 1); m.tz[lane] = wrap1(s, uz, 2);
-                m.ip[lane] = pl;
-                m.qe[lane] = ptab[pl.w & 0xffff];
-                g2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                const double rx = (double)mcx_wdiff(pl.x, p0x) * s.unit[0];
-                const double ry = (double)mcx_wdiff(pl.y, p0y) * s.unit[1];
-                const double rz = (double)mcx_wdiff(pl.z, p0z) * s.unit[2];
-                rad2 = rx * rx + ry * ry + rz * rz;
-            }
-            const double gm = sqrt(wave_max0(g2)), rm = sqrt(wave_max0(rad2));
-            // A translation, or a rotation whose atoms all lie within a
-            // quarter box of the pivot, moves the set rigidly: distances,
-            // angles and torsions inside it change by rounding only
-            // (~1e-15 relative), so the terms among its own atoms are
-            // skipped.  A rotation of a set the box edge splits (the pivot
-            // is applied to wrapped coordinates, include/Move.hpp:120-143)
-            // is not rigid and keeps them.
-            rigid = __ballot(far) == 0;
-            if (lane == 0) {
-                // |x_j - x_a| >= |x_j - x_0| - rm and a moves by at most gm, so
-                // a candidate farther than rc + rm + gm from atom 0's old
-                // position is outside the cutoff before and after the move;
-                // inside that sphere, o + g stays within the half box when
-                // rc + rm + 2 gm < L/2 on every axis
-                const double rc = sqrt(s.ctoff2) + rm + gm;
-                const double reach = rc + gm;
-                m.first = first; m.last = last; m.nm = nm;
-                m.thr = s.cutmode == 0 ? __builtin_huge_valf() : (float)(rc * rc * 1.00001 + 1e-4);
-                m.check = s.cutmode == 0 ||
-                          !(reach * 1.00001 < 0.5 * s.L[0] && reach * 1.00001 < 0.5 * s.L[1] &&
-                            reach * 1.00001 < 0.5 * s.L[2]);
-            }
+            // ---- wavefront 0: phase A, unless the move was read ahead
+            if (!ready) q = batch_pick<IMG>(s, mv, tab, dmax, ptab, f, ld, mb[c], sg, lane);
+            if (lane == 0) mcur = c;
             __syncthreads();
+            BMove &m = mb[c];
+            const int tp = q.tp, first = q.first, last = q.last, nm = q.nm;
+            const int ra0 = q.ra0, ra1 = q.ra1, pa0 = q.pa0, pa1 = q.pa1;
+            const bool rigid = q.rigid;
             // ---- dE: wavefronts 1.. scan the nonbonded pairs; wavefront 0
             // takes the bonded and 1-4 terms, removes excluded pairs the scan
             // counted, and evaluates pairs of two moved atoms
             double part = 0.0;
             double dec = 0.0, dev = 0.0, deb = 0.0;
             // moved atoms' old and trial coordinates from LDS, others from memory
-            const CoordFetch fo{s.x, s.y, s.z, m.ox, m.oy, m.oz, first, last};
+            const CoordFetch fo{s.x, s.y, s.z, sg.ox, sg.oy, sg.oz, first, last};
             const CoordFetch fn{s.x, s.y, s.z, m.tx, m.ty, m.tz, first, last};
             auto term = [&](const int4 &id, int kk, int ty_, const double4 &par) {
                 double e0, c0, v0, e1, c1, v1;
@@ -423,15 +464,15 @@ __global__ __launch_bounds__(NT) void k_batch_sweep(BatchArgs a)
             };
             for (int e = ra0 + lane; e < ra1; e += 64) {
                 const int k = e - ra0;
-                const int4 id = k < kBRec ? m.rid[k] : mv.rec_ids[e];
+                const int4 id = k < kBRec ? sg.rid[k] : mv.rec_ids[e];
                 if (rigid && inside(id)) continue;
-                if (k < kBRec) term(id, m.rkk[k], m.rty[k], m.rpar[k]);
+                if (k < kBRec) term(id, sg.rkk[k], sg.rty[k], sg.rpar[k]);
                 else term(id, mv.rec_kk[e], mv.rec_type[e], mv.rec_par[e]);
             }
             // pair corrections: excluded partners outside the moved range,
             // removed with the scan's own values, and pairs of two moved atoms
             for (int e = pa0 + lane; e < pa1; e += 64) {
-                const int2 pe = e - pa0 < kBRec ? m.pent[e - pa0] : mv.pair_ent[e];
+                const int2 pe = e - pa0 < kBRec ? sg.pent[e - pa0] : mv.pair_ent[e];
                 const int ia = pe.x & 0xffff;
                 const int4 p = m.ip[ia];
                 double c = 0.0, v = 0.0;
@@ -457,13 +498,25 @@ __global__ __launch_bounds__(NT) void k_batch_sweep(BatchArgs a)
             }
             part = MCX_CHGCHARMM * MCX_KCALTOIU * dec + 4.0 * dev + deb;
             part = wave_total(part);
-            if (lane == 0) m.red[tid >> 6] = part;
+            // ---- read ahead: the next move into the other buffer, valid if
+            // this one is rejected (then its acceptance draw was taken); the
+            // staged old coordinates and records of this move are no longer
+            // read
+            int nd = -1;
+            BRegs qn{};
+            if (mi + 1 < nmoves && !tab_adapts(tab, first_step + mi + 1)) {
+                PeekDraws pk{ld.d.v, ld.v1, ld.d.cur + 1, true};
+                const int i0 = pk.idx;
+                qn = batch_pick<IMG>(s, mv, tab, dmax, ptab, f, pk, mb[c ^ 1], sg, lane);
+                if (pk.ok) nd = pk.idx - i0;
+            }
+            if (lane == 0) sg.red[tid >> 6] = part;
             __syncthreads();
             // ---- wavefront 0: Metropolis test (apply_criterion,
             // src/MC_metropolis.cpp:175-204), bookkeeping and commit; the next
             // move's barrier publishes the commit
-            const double de = wave_total(lane < NW ? m.red[lane] : 0.0);
-            const int acc = metropolis(dr, beta, de, lane);
+            const double de = wave_total(lane < NW ? sg.red[lane] : 0.0);
+            const int acc = metropolis(ld, beta, de, lane);
             if (lane == 0) {
                 n_trial[tp]++;
                 if (acc) { etot += de; n_acc[tp]++; }
@@ -493,17 +546,29 @@ __global__ __launch_bounds__(NT) void k_batch_sweep(BatchArgs a)
                 s.ipos[i] = p;
                 if (IMG) img[i] = make_int4(p.x, p.y, p.z, tw);
             }
+            if (acc) {
+                ready = false;
+            } else if (nd >= 0) {
+                // the move read ahead is the next one: take its draws
+                for (int k = 0; k < nd; k++) (void)ld.next(lane);
+                c ^= 1;
+                q = qn;
+                ready = true;
+            } else {
+                ready = false;
+            }
         }
     } else {
         for (int mi = 0; mi < nmoves; mi++) {
             __syncthreads();              // the move is published
             double part = 0.0;
+            const BMove &m = mb[mcur];
             const int n_m = m.nm;
             if (n_m == 3) part = scan_pairs<NT, U, IMG, 3, QT>(s, m, f, ptab, queue, frozen_any, npair);
             else if (n_m == 1) part = scan_pairs<NT, U, IMG, 1, QT>(s, m, f, ptab, queue, frozen_any, npair);
             else part = scan_pairs<NT, U, IMG, 0, QT>(s, m, f, ptab, queue, frozen_any, npair);
             part = wave_total(part);
-            if (lane == 0) m.red[tid >> 6] = part;
+            if (lane == 0) sg.red[tid >> 6] = part;
             __syncthreads();
         }
     }
@@ -519,7 +584,7 @@ __global__ __launch_bounds__(NT) void k_batch_sweep(BatchArgs a)
             R.trials[lane] = n_trial[lane]; R.accepted[lane] = n_acc[lane];
         }
         if (lane == 0) { R.etot[0] = etot; R.step[0] = first_step + nmoves; }
-        dr.save(R.rng, lane);
+        ld.save(R.rng, lane);
     }
     if (R.rng_mode == 1)
         for (int k = tid; k < kMTN; k += NT) R.rng[k] = mt[k];

@@ -169,6 +169,82 @@ struct WaveDraws {
     }
 };
 
+// Read-ahead proposals.  MC_metropolis::apply_criterion draws its uniform
+// only when dE exceeds DBL_EPSILON (src/MC_metropolis.cpp:180-187), so a
+// rejected move always used its proposal draws and one acceptance draw and
+// left the state unchanged: the next move's proposal then starts at a draw
+// position known in advance.  The chain kernels draw and load it while the
+// current move is evaluated, from the two batches below.
+
+// The draw stream with the batch after the current one already generated,
+// so that up to 32 draws beyond the current batch can be read ahead.
+struct LookDraws {
+    WaveDraws d;   // current batch (d.v, d.cur) and the generator
+    double v1;     // the next batch
+
+    __device__ __forceinline__ void gen(int lane)
+    {
+        const double keep = d.v;
+        const int c = d.cur;
+        d.refill(lane);
+        v1 = d.v;
+        d.v = keep;
+        d.cur = c;
+    }
+    __device__ __forceinline__ void load(const uint32_t *st, int lane)
+    {
+        d.load(st, lane);
+        if (st[kRngWords - 1])
+            v1 = lane < 32 ? __hiloint2double((int)st[kRngAhead + 2 * lane + 1], (int)st[kRngAhead + 2 * lane]) : 0.0;
+        else
+            gen(lane);
+    }
+    __device__ __forceinline__ void save(uint32_t *st, int lane) const
+    {
+        d.save(st, lane);
+        if (lane < 32) {
+            st[kRngAhead + 2 * lane] = (uint32_t)__double2loint(v1);
+            st[kRngAhead + 2 * lane + 1] = (uint32_t)__double2hiint(v1);
+        }
+        if (lane == 0) st[kRngWords - 1] = 1u;
+    }
+    __device__ __forceinline__ double next(int lane)
+    {
+        if (d.cur == 32) {
+            d.v = v1;
+            d.cur = 0;
+            gen(lane);
+        }
+        const double u = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(d.v), d.cur),
+                                          __builtin_amdgcn_readlane(__double2loint(d.v), d.cur));
+        d.cur++;
+        return u;
+    }
+};
+
+// Read-ahead cursor over the two batches of a LookDraws (nothing consumed).
+// Past their end it returns 0.5 (which ends the unit-ball rejection loop)
+// and clears ok: that proposal is dropped.
+struct PeekDraws {
+    double v0, v1;
+    int idx;       // draw index from the start of the current batch
+    bool ok;
+    __device__ __forceinline__ double next(int)
+    {
+        double u = 0.5;
+        if (idx < 64) {
+            const double src = idx < 32 ? v0 : v1;
+            const int l = idx & 31;
+            u = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(src), l),
+                                 __builtin_amdgcn_readlane(__double2loint(src), l));
+        } else {
+            ok = false;
+        }
+        idx++;
+        return u;
+    }
+};
+
 __device__ __forceinline__ double wsum(double v)
 {
 #pragma unroll

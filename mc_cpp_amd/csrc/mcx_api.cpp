@@ -925,7 +925,7 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
         std::vector<double> dm(ntypes);
         for (int t = 0; t < ntypes; t++) dm[t] = types[t].dmax;
         dmax.upload(dm.data(), ntypes, h->stream);
-        std::vector<uint32_t> st(kRunRngWords, 0);
+        std::vector<uint32_t> st(kRngWords, 0);
         mt_seed(st.data(), seed);
         st[625] = 0; st[626] = 0;
         st[627] = 32;     // empty draw batch: the first draw refills it

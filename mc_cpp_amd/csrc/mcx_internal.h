@@ -36,11 +36,10 @@ constexpr int kFullTile = MCX_FULL_TILE;   // i and j atoms per tile of the O(N^
 constexpr int kMaxMoveTypes = 8;
 // Saved draw state of a chain (uint32 words): [0,624) MT19937 state, 624 MT
 // index, 625/626 Philox block counter (hi, lo), 627 cursor into the draw
-// batch, [628,692) the batch (32 doubles).
-constexpr int kRngWords = 692;
-// The single-chain state adds the batch after the current one, drawn ahead
-// for speculative proposals: [692,756) (32 doubles) and 756 (1: present).
-constexpr int kRunRngWords = 757;
+// batch, [628,692) the batch (32 doubles), [692,756) the batch after it,
+// drawn ahead for read-ahead proposals, and 756 (1: that batch is present).
+constexpr int kRngAhead = 692;
+constexpr int kRngWords = 757;
 
 // Everything a kernel needs about one system, passed by value.
 // Per-atom parameters of the all-pair kernel's j atom, read as one 16-byte

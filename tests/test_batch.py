@@ -113,10 +113,17 @@ def test_batch_replay_matches_oracle():
         assert abs(st["etot"][r] - ro["etot"]) <= 1e-6 * abs(ro["etot"]), (r, st["etot"][r], ro["etot"])
 
 
-def test_batch_matches_single_chain_loop():
+@pytest.mark.parametrize("adapt", [False, True])
+def test_batch_matches_single_chain_loop(adapt):
     """Philox streams: replica r of the batch and the single-chain device
-    loop on the same system with stream r make the same moves."""
+    loop on the same system with stream r make the same moves; with adapt,
+    dmax is adapted every 70 / 110 steps (both loops stop reading the next
+    move ahead at those steps)."""
     reps = [_water(n, 30 + k, 280.0 + 20 * k, 310 + k) for k, n in enumerate((150, 400, 1000))]
+    if adapt:
+        for s in reps:
+            for k, m in enumerate(s.moves):
+                m.each = 70 + 40 * k
     seed, n = 99, 600
     handles = [mcx.from_system(s.copy()) for s in reps]
     b = mcx.Batch(handles, reps[0].moves)
@@ -133,6 +140,9 @@ def test_batch_matches_single_chain_loop():
         xs, ys, zs = ff.get_coords()
         assert np.max(np.abs(xb - xs)) < 1e-12 and np.max(np.abs(zb - zs)) < 1e-12
         assert abs(st["etot"][r] - res["etot"]) <= 1e-9 * abs(res["etot"])
+        if adapt:
+            assert np.array_equal(st["dmax"][r][: len(s.moves)], res["dmax"])
+            assert not np.array_equal(res["dmax"], np.array([m.dmax for m in s.moves]))
 
 
 def test_batch_launch_shapes_agree():
