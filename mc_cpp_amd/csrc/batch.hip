@@ -133,7 +133,8 @@ __device__ __forceinline__ bool pair_moved_fixed(const DevSys &s, const int4 &p,
     const double oy = (double)mcx_wdiff(pj.y, p.y) * s.unit[1];
     const double oz = (double)mcx_wdiff(pj.z, p.z) * s.unit[2];
     double c, v;
-    if (mcx_pair_change(s, ox, oy, oz, -gx, -gy, -gz, qa.x, qa.y, qa.z, P.x, P.y, P.z, check, c, v)) {
+    const bool lj = __ballot(qa.y != 0.0) != 0ull;   // some moved atom here has eps != 0
+    if (mcx_pair_change(s, ox, oy, oz, -gx, -gy, -gz, qa.x, qa.y, qa.z, P.x, P.y, P.z, check, c, v, lj)) {
         dec += c;
         dev += v;
         return true;

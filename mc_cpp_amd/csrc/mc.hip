@@ -85,7 +85,10 @@ __device__ __forceinline__ void pair_delta(const DevSys &s, const MoveLDS &m, in
     if (b >= 0) { g[0] += m.dx[b]; g[1] += m.dy[b]; g[2] += m.dz[b]; }
     double c, v;
     const double4 qa = m.mq[a];
-    if (mcx_pair_change(s, o[0], o[1], o[2], g[0], g[1], g[2], qa.x, qa.y, qa.z, pq.x, pq.y, pq.z, true, c, v)) {
+    // no Lennard-Jones change when every moved atom of this wavefront's
+    // entries has eps = 0 (the hydrogens of a water)
+    const bool lj = __ballot(qa.y != 0.0) != 0ull;
+    if (mcx_pair_change(s, o[0], o[1], o[2], g[0], g[1], g[2], qa.x, qa.y, qa.z, pq.x, pq.y, pq.z, true, c, v, lj)) {
         dec += c;
         dev += v;
     }
@@ -266,9 +269,12 @@ __device__ __forceinline__ void delta_terms(const DevSys &s, const DevMoves &mv,
 }
 
 // Workgroup reduction of the parts: m.de (and m.fnew with fb) valid after
-// the final barrier.
+// the final barrier.  With wave0_only, m.de is needed by wavefront 0 alone:
+// the final workgroup barrier becomes a wavefront barrier (the other
+// wavefronts write partials again only after the next step's first
+// barrier, when wavefront 0 has read these).
 __device__ __forceinline__ void delta_finish(MoveLDS &m, bool fb, double dec, double dev, double deb,
-                                             double (&fnew)[3])
+                                             double (&fnew)[3], bool wave0_only = false)
 {
     const int tid = threadIdx.x;
     dec = wsum(dec);
@@ -291,7 +297,8 @@ __device__ __forceinline__ void delta_finish(MoveLDS &m, bool fb, double dec, do
         for (int k = 0; k < kMCW; k++) v += m.red[2 + tid][k];
         m.fnew[tid - 1] = v;
     }
-    __syncthreads();
+    if (wave0_only) wave_sync();
+    else __syncthreads();
 }
 
 // All three parts (the one-shot dE kernel).
@@ -673,7 +680,7 @@ struct PipeLDS {
 
 // The Metropolis loop with read-ahead proposals, for move sets without
 // force-bias moves; same arguments and the same chain as k_mc_chunk, and
-// the same three workgroup barriers per step in both role loops.
+// the same two workgroup barriers per step in both role loops.
 template <int PB, int CUT>
 __global__ __launch_bounds__(kMCT) void k_mc_spec(DevSys s0, DevTerms t, DevMoves mv, MCRunArgs arg)
 {
@@ -738,7 +745,7 @@ __global__ __launch_bounds__(kMCT) void k_mc_spec(DevSys s0, DevTerms t, DevMove
                 pick_load(s, mv, pk, dmax, g.m[cur ^ 1], lane, arg.ipos_ref, arg.half_skin2);
                 if (pk.ok) nd = pk.idx - i0;
             }
-            delta_finish(m, false, dec, dev, deb, fnew);
+            delta_finish(m, false, dec, dev, deb, fnew, true);
             // Metropolis test (apply_criterion, src/MC_metropolis.cpp:175-204),
             // bookkeeping, commit of the moved atoms
             const int st = arg.step0 + step + 1;
@@ -791,7 +798,7 @@ __global__ __launch_bounds__(kMCT) void k_mc_spec(DevSys s0, DevTerms t, DevMove
             double dec = 0.0, dev = 0.0;
             double fnew[3] = {0.0, 0.0, 0.0};
             delta_pairs(s, m, false, dec, dev, fnew);
-            delta_finish(m, false, dec, dev, 0.0, fnew);
+            delta_finish(m, false, dec, dev, 0.0, fnew, true);
         }
     }
     if (arg.stop_at && tid == 0) {

@@ -440,10 +440,12 @@ __device__ __forceinline__ void mcx_pair_force(const DevSys &s, const double d[3
 // from the FP32 estimate and one FP64 Newton step (~1e-14).  Returns false
 // (and adds nothing) when neither distance is inside the cutoff; otherwise
 // c = q_i*dC and v = eps_i*dV, the raw sums before the 332.0716*418.4 and
-// 4 factors of src/FField_MDBAS.cpp:420-421.
+// 4 factors of src/FField_MDBAS.cpp:420-421.  With lj false (a
+// wavefront-uniform flag, set only when ei is zero on every lane of the
+// call) the Lennard-Jones part is skipped: v = ei * 0 adds the same zero.
 __device__ __forceinline__ bool mcx_pair_change(const DevSys &s, double ox, double oy, double oz, double gx, double gy,
                                                 double gz, double qi, double ei, double si, double qj, double ej,
-                                                double sj, bool check, double &c, double &v)
+                                                double sj, bool check, double &c, double &v, bool lj = true)
 {
     double dn[3] = {ox + gx, oy + gy, oz + gz};
     const bool full = s.cutmode == 0;
@@ -458,12 +460,15 @@ __device__ __forceinline__ bool mcx_pair_change(const DevSys &s, double ox, doub
     const double rio = mcx_rsqrt(r2o, f32), rin = mcx_rsqrt(r2n, f32);
     const double So = ino ? (full ? 1.0 : mcx_switch(s, r2o)) : 0.0;
     const double Sn = inn ? (full ? 1.0 : mcx_switch(s, r2n)) : 0.0;
-    const double sij = si + sj;
-    const double s2 = sij * sij;
-    const double to = s2 * (rio * rio), tn = s2 * (rin * rin);
-    const double t3o = to * to * to, t3n = tn * tn * tn;
     const double dc = qj * fma(Sn, rin, -(So * rio));
-    const double dv = ej * fma(Sn, fma(t3n, t3n, -t3n), -(So * fma(t3o, t3o, -t3o)));
+    double dv = 0.0;
+    if (lj) {
+        const double sij = si + sj;
+        const double s2 = sij * sij;
+        const double to = s2 * (rio * rio), tn = s2 * (rin * rin);
+        const double t3o = to * to * to, t3n = tn * tn * tn;
+        dv = ej * fma(Sn, fma(t3n, t3n, -t3n), -(So * fma(t3o, t3o, -t3o)));
+    }
     c = qi * dc;
     v = ei * dv;
     return true;
