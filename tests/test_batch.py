@@ -145,6 +145,29 @@ def test_batch_matches_single_chain_loop(adapt):
             assert not np.array_equal(res["dmax"], np.array([m.dmax for m in s.moves]))
 
 
+def test_batch_without_lds_image_matches_single_chain_loop():
+    """A replica of 9,900 atoms does not fit in the LDS image, so the whole
+    batch runs the variant that reads positions from device memory; it
+    still makes the single-chain loop's moves."""
+    reps = [_water(n, 40 + k, 290.0 + 10 * k, 320 + k) for k, n in enumerate((150, 3300))]
+    seed, n = 77, 400
+    handles = [mcx.from_system(s.copy()) for s in reps]
+    b = mcx.Batch(handles, reps[0].moves)
+    b.reset(seed, mcx.MC_PHILOX)
+    acc, de = b.moves(n)
+    st = b.state()
+    for r, s in enumerate(reps):
+        ff = mcx.FField(s.copy())
+        ff.h.set_rng_stream(r)
+        res = ff.h.mc_run(s.moves, n, seed, rng_mode=mcx.MC_PHILOX, list_every=50)
+        n_ok = _compare(acc[r], de[r], res["acc"], res["de"], _tol(s, 1e-9), "replica %d" % r, DE_CMP)
+        assert n_ok == n, ("replica %d diverged at move %d" % (r, n_ok))
+        xb, yb, zb = handles[r].get_coords()
+        xs, ys, zs = ff.get_coords()
+        assert np.max(np.abs(xb - xs)) < 1e-12 and np.max(np.abs(zb - zs)) < 1e-12
+        assert abs(st["etot"][r] - res["etot"]) <= 1e-9 * abs(res["etot"])
+
+
 def test_batch_launch_shapes_agree():
     reps = [_water(n, 40 + k, 300.0, 320 + k) for k, n in enumerate((120, 500, 2500))]
     coords = [(s.x.copy(), s.y.copy(), s.z.copy()) for s in reps]
