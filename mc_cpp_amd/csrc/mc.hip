@@ -778,14 +778,22 @@ __global__ __launch_bounds__(kMCT) void k_mc_spec(DevSys s0, DevTerms t, DevMove
                     far = arg.ipos_ref && beyond_half_skin(s, np, arg.ipos_ref[i], arg.half_skin2);
                 }
                 if (__ballot(far) != 0ull) stop = true;
-                ready = false;
-            } else if (nd >= 0) {
+                // An acceptance that took its acceptance draw (dE above
+                // DBL_EPSILON) leaves the stream where the read-ahead
+                // assumed; the proposal stays valid if none of its atoms or
+                // its pivot moved (and the chunk goes on: a move that left
+                // the list's validity ends it, and the next chunk draws anew).
+                const MoveLDS &mn = g.m[cur ^ 1];
+                const bool apart = mn.last < m.first || mn.first > m.last;
+                const bool pivot_ok = mn.pivot < 0 || mn.pivot < m.first || mn.pivot > m.last;
+                ready = !stop && nd >= 0 && de > DBL_EPSILON && apart && pivot_ok;
+            } else {
+                ready = nd >= 0;
+            }
+            if (ready) {
                 // the read-ahead proposal is the next move: take its draws
                 for (int q = 0; q < nd; q++) (void)ld.next(lane);
                 cur ^= 1;
-                ready = true;
-            } else {
-                ready = false;
             }
             wave_sync();
         }
