@@ -658,7 +658,9 @@ __global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s0, DevTerms t, DevMov
 // while the other wavefronts evaluate move k's pairs, wavefront 0 draws and
 // loads move k+1 into a second buffer on the assumption that move k is
 // rejected (about four moves in five).  If it is, the next step starts
-// from that buffer at once; if move k is accepted, the buffer is dropped
+// from that buffer at once.  If move k is accepted, the buffer is kept
+// only when the acceptance took its draw (dE above DBL_EPSILON) and moved
+// none of the buffered move's atoms or its pivot; otherwise it is dropped
 // (nothing was consumed from the stream) and move k+1 is drawn again after
 // the commit.  The chain is the one k_mc_chunk produces.  No read-ahead is
 // done across a step after which dmax is adapted.
