@@ -52,7 +52,7 @@ COPY_BYTES = 256 << 20         # calibration copy of the counter runs
 
 def cpu_reference_baseline(s, fp, nmoves, seed):
     """Times the reference's MC_metropolis::run (two full getE per move) on
-    a bounded sample; returns (moves_per_s, seconds, threads)."""
+    a bounded sample on one core; returns (moves_per_s, seconds)."""
     from oracle import oracle
     from mc_cpp_amd import systems
     if not oracle.ref_available():
