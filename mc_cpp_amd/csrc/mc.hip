@@ -132,7 +132,7 @@ __device__ __forceinline__ void force_old(const DevSys &s, MoveLDS &m)
         const double4 qa = m.mq[lo], pq = s.qes[j];
         mcx_pair_force(s, d, qa.x, qa.y, qa.z, pq.x, pq.y, pq.z, f);
     }
-    for (int c = 0; c < 3; c++) f[c] = wsum(f[c]);
+    for (int c = 0; c < 3; c++) f[c] = wsum_dpp(f[c]);
     const int w = tid >> 6;
     if ((tid & 63) == 0) { m.red[3][w] = f[0]; m.red[4][w] = f[1]; m.red[5][w] = f[2]; }
     __syncthreads();
@@ -277,13 +277,13 @@ __device__ __forceinline__ void delta_finish(MoveLDS &m, bool fb, double dec, do
                                              double (&fnew)[3], bool wave0_only = false)
 {
     const int tid = threadIdx.x;
-    dec = wsum(dec);
-    dev = wsum(dev);
-    deb = wsum(deb);
+    dec = wsum_dpp(dec);
+    dev = wsum_dpp(dev);
+    deb = wsum_dpp(deb);
     const int w = tid >> 6;
     if ((tid & 63) == 0) { m.red[0][w] = dec; m.red[1][w] = dev; m.red[2][w] = deb; }
     if (fb) {
-        for (int c = 0; c < 3; c++) fnew[c] = wsum(fnew[c]);
+        for (int c = 0; c < 3; c++) fnew[c] = wsum_dpp(fnew[c]);
         if ((tid & 63) == 0) { m.red[3][w] = fnew[0]; m.red[4][w] = fnew[1]; m.red[5][w] = fnew[2]; }
     }
     __syncthreads();

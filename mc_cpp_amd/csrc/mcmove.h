@@ -245,11 +245,37 @@ struct PeekDraws {
     }
 };
 
+// One DPP move of an FP64 value (both halves; lanes the pattern or the
+// row mask leaves out receive 0).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp64(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
 __device__ __forceinline__ double wsum(double v)
 {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
+}
+
+// The same sum by DPP register moves instead of LDS round trips, for code
+// that every lane of the wavefront reaches: quad swaps, half-row and row
+// mirrors (every lane of a row then holds the row's sum), row_bcast15 and
+// row_bcast31 carry the row sums into lane 63, which is broadcast.
+__device__ __forceinline__ double wsum_dpp(double v)
+{
+    v += dpp64<0xB1, 0xf>(v);    // quad_perm [1,0,3,2]
+    v += dpp64<0x4E, 0xf>(v);    // quad_perm [2,3,0,1]
+    v += dpp64<0x141, 0xf>(v);   // row_half_mirror
+    v += dpp64<0x140, 0xf>(v);   // row_mirror
+    v += dpp64<0x142, 0xa>(v);   // row_bcast15 into rows 1 and 3
+    v += dpp64<0x143, 0xc>(v);   // row_bcast31 into rows 2 and 3
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 63),
+                            __builtin_amdgcn_readlane(__double2loint(v), 63));
 }
 
 __device__ __forceinline__ double wmax(double v)
