@@ -1027,6 +1027,7 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
             if (b != kNone) {
                 done = b;
                 build_list(h);               // grows the capacity until the rows fit
+                nlist++;
                 bail_d.upload(&kNone, 1, h->stream);
             }
             since_check = 0;
@@ -1043,12 +1044,27 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
             if (rebuilds && done % list_every == 0) {
                 launch_list_build(h->sys(), h->grid, h->stream);
                 HIPCHK(hipGetLastError());
+                nlist++;
             }
             const bool frame = save && done % save == 0;
+            const int reached = done;
             if (frame || ++since_check == kCheckEvery || done >= nsteps) check_bail();
-            if (frame && done % save == 0) emit_frame(done);
+            // a rewind (done < reached) resumes at a step whose frame, if
+            // any, was written when it was first reached
+            if (frame && done == reached) emit_frame(done);
         }
-        if (rebuilds) nlist += nsteps / list_every;
+        if (rebuilds && done >= nsteps) {
+            // a rebuild queued after the last chunk has no chunk after it to
+            // notice an overflow: grow the list here, so that later energy and
+            // gradient calls never see truncated rows
+            int ov = 0;
+            HIPCHK(hipMemcpyAsync(&ov, h->overflow.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+            if (ov) {
+                build_list(h);
+                nlist++;
+            }
+        }
         HIPCHK(hipStreamSynchronize(h->stream));
         const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (stats) {

@@ -545,3 +545,145 @@ def mixed_system(seed: int = 3, nside: int = 32, L: float = 99.4, nbead: int = 2
     s.moves = [MoveType(MV_TRN, MODE_RESIDUE, 0.3, 50.0, 0, sel_w), MoveType(MV_ROT, MODE_RESIDUE, 15.0, 50.0, 0, sel_w),
                MoveType(MV_TRN, MODE_ATOM, 0.1, 50.0, 0, sel_c)]
     return s, fp
+
+
+def bonded_mix(seed: int = 7, nchain: int = 6, nbead: int = 16, box=(26.3, 27.9, 25.1), sites=(8, 9, 8),
+               jitter: float = 0.08, clash: float = 3.0, cutmode: int = CUT_SWITCH, pbtype: int = PBC_ORBIC):
+    """Every bonded term kind the reference reads, in one periodic box.
+
+    Waters (lattice sites `sites` over an orthorhombic `box`, seeded random
+    orientations; waters clashing with a bead removed) plus `nchain` flexible
+    bead chains of `nbead` beads, one residue each, built by seeded random
+    walks and jittered by U(+-jitter) per coordinate so that no term sits at
+    its minimum.  Chain terms, in file units:
+      * bonds alternate harmonic (type 0, k 300, r0 1.53) and Morse
+        (type 1, well depth 80 kcal/mol, beta 1.8 1/A, r0 1.53);
+      * angles theta0 111 deg, k 50, each with a Urey-Bradley 1-3 term
+        (k 30, r0 2.52);
+      * dihedrals alternate cosine (type 1, k 0.2, n 3, phi0 0) and harmonic
+        (type 2, k 5, phi0 60 deg);
+      * impropers on every fourth bead alternate cosine (type 1, k 1, n 2,
+        phi0 180) and harmonic (type 2, k 10, phi0 0);
+      * charges U(-0.3, 0.3) made neutral per chain, sigma U[3, 4],
+        eps U[0.05, 0.2]; 1-4 parameters eps/2 and 0.9 sigma.
+    Moves: water residue TRN 0.3 A and ROT 15 deg, chain-bead TRN 0.12 A
+    (ATOM mode), whole-chain ROT 5 deg (RESIDUE mode)."""
+    bx = np.asarray(box, dtype=float)
+    ns = np.asarray(sites, dtype=int)
+    rng_u = philox.uniform(seed, 61, 4 * nchain * nbead * 64)
+    cur = 0
+    chains = []
+    th0 = np.radians(111.0)
+    for c in range(nchain):
+        p = np.zeros((nbead, 3))
+        # first two beads: drawn again while either comes closer than 4 A
+        # to a bead of an earlier chain
+        prev = np.concatenate(chains) if chains else np.zeros((0, 3))
+        for attempt in range(64):
+            u0 = philox.uniform(seed, 62 + 64 * c + attempt, 6)
+            p[0] = (u0[:3] - 0.5) * bx
+            d0 = u0[3:] - 0.5
+            p[1] = p[0] + 1.53 * d0 / np.linalg.norm(d0)
+            if len(prev) == 0:
+                break
+            dd = prev[None, :, :] - p[None, :2, :].transpose(1, 0, 2)
+            dd -= bx * np.round(dd / bx)
+            if np.min(np.sum(dd * dd, -1)) > 4.0 ** 2:
+                break
+        for i in range(2, nbead):
+            b = p[i - 1] - p[i - 2]
+            b /= np.linalg.norm(b)
+            tmp = np.array([1.0, 0, 0]) if abs(b[0]) < 0.9 else np.array([0, 1.0, 0])
+            p1 = np.cross(b, tmp); p1 /= np.linalg.norm(p1)
+            p2 = np.cross(b, p1)
+            # torsion drawn uniformly, drawn again (next draws) while the new
+            # bead comes closer than 4 A to an earlier bead of any chain
+            # other than its three predecessors (minimum image)
+            others = np.concatenate(chains + [p[:max(i - 3, 0)]])
+            for attempt in range(64):
+                phi = 2 * np.pi * rng_u[cur]
+                cur += 1
+                d = -np.cos(th0) * b + np.sin(th0) * (np.cos(phi) * p1 + np.sin(phi) * p2)
+                cand = p[i - 1] + 1.53 * d
+                if len(others) == 0:
+                    break
+                dd = others - cand
+                dd -= bx * np.round(dd / bx)
+                if np.min(np.sum(dd * dd, 1)) > 4.0 ** 2:
+                    break
+            p[i] = cand
+        chains.append(p)
+    beads = np.concatenate(chains)
+    nb = len(beads)
+    beads = beads + (2.0 * philox.uniform(seed, 90, 3 * nb).reshape(nb, 3) - 1.0) * jitter
+    # water lattice
+    g = np.stack(np.meshgrid(np.arange(ns[0]), np.arange(ns[1]), np.arange(ns[2]), indexing="ij"), -1).reshape(-1, 3)
+    O = (g + 0.5) * (bx / ns) - bx / 2
+    half = np.radians(THETA_HOH) / 2
+    body = np.array([[R_OH * np.sin(half), R_OH * np.cos(half), 0.0],
+                     [-R_OH * np.sin(half), R_OH * np.cos(half), 0.0]])
+    Rm = _random_rotations(seed, 91, len(O))
+    W = np.concatenate([O[:, None, :], O[:, None, :] + np.einsum("nij,kj->nki", Rm, body)], 1)
+    keep = np.ones(len(W), dtype=bool)
+    for a in range(3):
+        d = W[:, a, None, :] - beads[None, :, :]
+        d -= bx * np.round(d / bx)
+        keep &= ~(np.min(np.einsum("ijk,ijk->ij", d, d), 1) < clash * clash)
+    W = W[keep]
+    nw = len(W)
+    n = 3 * nw + nb
+    off = 3 * nw
+    qc = 0.6 * philox.uniform(seed, 92, nb) - 0.3
+    qc = (qc.reshape(nchain, nbead) - qc.reshape(nchain, nbead).mean(1, keepdims=True)).reshape(-1)
+    sg = 3.0 + philox.uniform(seed, 93, nb)
+    ep = 0.05 + 0.15 * philox.uniform(seed, 94, nb)
+    s = System(natom=n, box=bx.copy(), pbtype=pbtype, cutmode=cutmode, temp=300.0, name="bonded_mix",
+               charge=np.concatenate([np.tile([Q_O, Q_H, Q_H], nw), qc]),
+               mass=np.concatenate([np.tile([M_O, M_H, M_H], nw), np.full(nb, 12.011)]),
+               resid=np.concatenate([np.repeat(np.arange(1, nw + 1), 3),
+                                     np.repeat(np.arange(nw + 1, nw + 1 + nchain), nbead)]))
+    allpos = np.concatenate([W.reshape(-1, 3), beads])
+    s.x, s.y, s.z = wrap(s, allpos[:, 0].copy(), allpos[:, 1].copy(), allpos[:, 2].copy())
+    s.symbols = ["OH2", "H1", "H2"] * nw + ["CB"] * nb
+    s.resnames = ["TIP3"] * (3 * nw) + ["CHN"] * nb
+    wb = 3 * np.arange(nw)
+    ci = (off + np.arange(nb)).reshape(nchain, nbead)
+    b_a = np.concatenate([ci[:, :-1].reshape(-1)])
+    b_b = np.concatenate([ci[:, 1:].reshape(-1)])
+    nbb = len(b_a)
+    morse = (np.tile(np.arange(nbead - 1), nchain) % 2 == 1)
+    s.bonds = make_terms("bonds", a=np.concatenate([wb, wb, wb + 1, b_a]), b=np.concatenate([wb + 1, wb + 2, wb + 2, b_b]),
+                         type=np.concatenate([np.zeros(3 * nw, np.int32), morse.astype(np.int32)]),
+                         r0=np.concatenate([np.full(2 * nw, R_OH), np.full(nw, 2 * R_OH * np.sin(half)),
+                                            np.full(nbb, 1.53)]),
+                         beta=np.concatenate([np.zeros(3 * nw), np.where(morse, 1.8, 0.0)]))
+    a_a, a_b, a_c = ci[:, :-2].reshape(-1), ci[:, 1:-1].reshape(-1), ci[:, 2:].reshape(-1)
+    na = len(a_a)
+    s.angles = make_terms("angles", a=np.concatenate([wb + 1, a_a]), b=np.concatenate([wb, a_b]),
+                          c=np.concatenate([wb + 2, a_c]), type=np.zeros(nw + na, dtype=np.int32))
+    s.ub = make_terms("ub", a=a_a, b=a_c, r0=np.full(na, 2.52))
+    d_a, d_b, d_c, d_d = (ci[:, q:nbead - 3 + q].reshape(-1) for q in range(4))
+    nd = len(d_a)
+    dtype_ = np.where(np.tile(np.arange(nbead - 3), nchain) % 2 == 0, 1, 2).astype(np.int32)
+    s.dihedrals = make_terms("dihedrals", a=d_a, b=d_b, c=d_c, d=d_d, type=dtype_, order=np.ones(nd, np.int32),
+                             mult=np.where(dtype_ == 1, 3.0, 0.0))
+    ii = ci[:, 1:-2:4].reshape(-1)
+    ni = len(ii)
+    itype = np.where(np.arange(ni) % 2 == 0, 1, 2).astype(np.int32)
+    s.impropers = make_terms("impropers", a=ii, b=ii - 1, c=ii + 1, d=ii + 2, type=itype,
+                             order=np.ones(ni, np.int32), mult=np.where(itype == 1, 2.0, 0.0))
+    eps_k = np.concatenate([np.tile([EPS_O, 0.0, 0.0], nw), ep])
+    sig_h = np.concatenate([np.tile([SIG_O / 2, 0.0, 0.0], nw), sg / 2])
+    fp = dict(eps_kcal=eps_k, sig_half=sig_h, eps14_kcal=0.5 * eps_k, sig14_half=0.9 * sig_h,
+              bond_k_kcal=np.concatenate([np.full(2 * nw, K_OH), np.zeros(nw), np.where(morse, 80.0, 300.0)]),
+              ub_k_kcal=np.full(na, 30.0),
+              ang_k_kcal=np.concatenate([np.full(nw, K_HOH), np.full(na, 50.0)]),
+              ang_t0_deg=np.concatenate([np.full(nw, THETA_HOH), np.full(na, 111.0)]),
+              dih_k_kcal=np.where(dtype_ == 1, 0.2, 5.0), dih_phi0_deg=np.where(dtype_ == 1, 0.0, 60.0),
+              imp_k_kcal=np.where(itype == 1, 1.0, 10.0), imp_phi0_deg=np.where(itype == 1, 180.0, 0.0))
+    _from_file_units(s, fp)
+    sel_w = np.zeros(n, dtype=np.int32); sel_w[:off] = 1
+    sel_c = np.zeros(n, dtype=np.int32); sel_c[off:] = 1
+    s.moves = [MoveType(MV_TRN, MODE_RESIDUE, 0.3, 50.0, 0, sel_w), MoveType(MV_ROT, MODE_RESIDUE, 15.0, 50.0, 0, sel_w),
+               MoveType(MV_TRN, MODE_ATOM, 0.12, 50.0, 0, sel_c), MoveType(MV_ROT, MODE_RESIDUE, 5.0, 50.0, 0, sel_c)]
+    return s, fp

@@ -79,7 +79,10 @@ def main():
              "water4096_seed2": systems.water_box(seed=2),
              # a box smaller than twice the cutoff (replica-ladder sizes)
              "water90_small_box": systems.water_box(seed=21, nside=5, L=(90 / 0.0334) ** (1 / 3), nmol=90,
-                                                    temp=290.0, tag=300)}
+                                                    temp=290.0, tag=300),
+             # orthorhombic box with harmonic and Morse bonds, Urey-Bradley,
+             # cosine and harmonic dihedrals and impropers, 1-4 pairs
+             "bonded_mix_seed7": systems.bonded_mix(seed=7)}
     for name, (s, fp) in synth.items():
         cor, ff = systems.save_mdbas(os.path.join(tmp, name), s, fp)
         r = oracle.Reference(cor, ff, s)
@@ -87,7 +90,7 @@ def main():
         s2 = s.copy(); s2.cutmode = systems.CUT_FULL
         cor2, ff2 = systems.save_mdbas(os.path.join(tmp, name + "_full"), s2, fp)
         ent["energy_full"] = oracle.Reference(cor2, ff2, s2).getE().tolist()
-        if name != "water4096_seed2":
+        if name in ("argon_fcc_seed1", "water512_seed3", "water90_small_box"):
             nsteps = 1000 if name.startswith("argon") else 300
             seed = 12345 if name.startswith("argon") else 11 if name == "water90_small_box" else 777
             mc = r.mc_run(s.moves, nsteps, seed)

@@ -11,6 +11,7 @@ Tolerances (stated per quantity):
     whose dE lies within the dE tolerance of the acceptance threshold
     (after such a move the two chains legitimately diverge).
 """
+import json
 import os
 
 import numpy as np
@@ -18,6 +19,8 @@ import pytest
 
 from mc_cpp_amd import mcx, systems
 from oracle.oracle import Oracle
+
+GOLD = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "golden.json")))
 
 pytestmark = pytest.mark.gpu
 
@@ -49,6 +52,9 @@ SYSTEMS = {
     "argon864": lambda: systems.argon_fcc(seed=1)[0],
     "water1536": lambda: systems.water_box(seed=3, nside=8, L=24.85)[0],
     "mixed_small": lambda: systems.mixed_system(seed=3, nside=8, L=24.85, nbead=60)[0],
+    # orthorhombic; harmonic and Morse bonds, Urey-Bradley, cosine and
+    # harmonic dihedrals and impropers, 1-4 pairs with their own parameters
+    "bonded_mix": lambda: systems.bonded_mix(seed=7)[0],
 }
 
 
@@ -72,6 +78,54 @@ def test_full_energy(name):
     eo = o.getE()
     ff = mcx.FField(s)
     _check_energy(ff.getE10(), eo, name + "/full")
+
+
+def test_bonded_mix_against_reference_goldens():
+    """Every bonded kind against the reference's own numbers (the mc_cpp
+    sources built in oracle/_ref, recorded in golden.json), switched and
+    all-pair, in the orthorhombic box."""
+    g = GOLD["bonded_mix_seed7"]
+    for cut, key in ((systems.CUT_SWITCH, "energy"), (systems.CUT_FULL, "energy_full")):
+        s = systems.bonded_mix(seed=7)[0]
+        s.cutmode = cut
+        e = mcx.FField(s).getE10()
+        _check_energy(e, np.array(g[key]), "bonded_mix/" + key)
+        assert abs(e[5]) > 1e4 and abs(e[7]) > 1e3 and abs(e[8]) > 1e4 and abs(e[9]) > 1e4
+
+
+@pytest.mark.parametrize("name", ["nma_solvated", "trhbn_solvated"])
+def test_reference_solvated_systems(name, tmp_path):
+    """The reference's test/nma_solvated (2,658 atoms, cubic) and
+    test/trhbn_solvated (21,977 atoms, orthorhombic, 1,777 Urey-Bradley
+    terms), all-pair as their decks run them, against the reference's
+    energies; then switched (10/12 A, 2 A skin) against the FP64 oracle."""
+    from tests.refdata import reference_system
+    s, g = reference_system(name, tmp_path)
+    assert s.cutmode == systems.CUT_FULL
+    e = mcx.FField(s.copy()).getE10()
+    _check_energy(e, np.array(g["energy"]), name + "/full")
+    if name == "trhbn_solvated":
+        assert len(s.ub["a"]) == 1777 and abs(e[7]) > 0.0
+    s.cutmode = systems.CUT_SWITCH
+    o, ref = _pair(s)
+    o.update_list()
+    ff = mcx.FField(s)
+    _check_list(ff.h, o)
+    _check_energy(ff.getE10(), o.getE(), name + "/switch")
+
+
+def test_morse_bond_closed_form():
+    """A single Morse bond on the device (src/FField_MDBAS.cpp:519-527):
+    E = D (1 - e^{-b (d - r0)})^2, zero at r0."""
+    D, b, r0 = 80.0 * 418.4, 1.8, 1.53
+    for d in (1.53, 1.2, 1.9, 3.0):
+        s = systems.System(natom=2, box=[50.0] * 3, pbtype=systems.PBC_NONE, cutmode=systems.CUT_FULL,
+                           x=[0.0, d], y=[0.0, 0.0], z=[0.0, 0.0], resid=[1, 1])
+        s.bonds = systems.make_terms("bonds", a=[0], b=[1], type=[1], k=[D], r0=[r0], beta=[b])
+        e = mcx.FField(s).getE10()
+        want = D * (1.0 - np.exp(-b * (d - r0))) ** 2
+        assert abs(e[5] - want) <= 1e-12 * D, (d, e[5], want)
+        assert abs(e[0] - want) <= 1e-12 * D
 
 
 def test_water_box_12288_energy_and_list():
@@ -151,19 +205,30 @@ def _kT(s):
     return 0.83144621 * s.temp
 
 
-@pytest.mark.parametrize("name", ["argon864", "water1536", "mixed_small"])
+@pytest.mark.parametrize("name", ["argon864", "water1536", "mixed_small", "bonded_mix", "bonded_mix_beads",
+                                  "bonded_mix_full"])
 def test_delta_energy(name):
-    s = SYSTEMS[name]()
+    """Residue translations (bonded_mix_beads: single chain beads, whose
+    bonds, angles, Urey-Bradley terms, torsions and 1-4 pairs change;
+    bonded_mix_full: all-pair in the orthorhombic box)."""
+    s = SYSTEMS[name.replace("_beads", "").replace("_full", "")]()
+    if name.endswith("_full"):
+        s.cutmode = systems.CUT_FULL
     o, ref = _pair(s)
-    o.update_list()
+    if s.cutmode == systems.CUT_SWITCH:
+        o.update_list()
     ff = mcx.FField(s)
     rng = np.random.default_rng(11)
     x, y, z = ref.x, ref.y, ref.z
+    beads = np.nonzero(np.asarray(s.resnames) == "CHN")[0] if name.startswith("bonded") else None
     for trial in range(30):
-        i = int(rng.integers(0, s.natom))
-        r = s.resid[i]
-        idx = np.nonzero(s.resid == r)[0]
-        a, b = int(idx[0]), int(idx[-1])
+        if name.endswith("_beads"):
+            a = b = int(rng.choice(beads))
+        else:
+            i = int(rng.choice(beads)) if (beads is not None and trial % 2) else int(rng.integers(0, s.natom))
+            r = s.resid[i]
+            idx = np.nonzero(s.resid == r)[0]
+            a, b = int(idx[0]), int(idx[-1])
         d = rng.uniform(-0.3, 0.3, 3)
         nx, ny, nz = x[a:b + 1] + d[0], y[a:b + 1] + d[1], z[a:b + 1] + d[2]
         de_g = ff.h.delta_energy(a, b, nx, ny, nz)
@@ -295,6 +360,33 @@ def test_mc_read_ahead_same_chain(case, monkeypatch):
         assert not np.array_equal(r0["dmax"], np.array([m.dmax for m in s.moves]))
 
 
+@pytest.mark.parametrize("cut", ["switch", "full"])
+def test_mc_replay_bonded_mix(cut):
+    """2,000 replayed moves on the orthorhombic system with every bonded
+    kind: water residue TRN/ROT, single chain beads (Morse and harmonic
+    bonds, angles, Urey-Bradley, both torsion kinds, 1-4 pairs change) and
+    whole-chain rotations (16 moved atoms)."""
+    s = systems.bonded_mix(seed=7)[0]
+    if cut == "full":
+        s.cutmode = systems.CUT_FULL
+    o, ref = _pair(s)
+    if s.cutmode == systems.CUT_SWITCH:
+        o.update_list()
+    e0 = abs(o.getE()[0])
+    ff = mcx.FField(s)
+    n = 2000 if cut == "switch" else 800
+    res = mcx.MC_metropolis(ff, nsteps=n, seed=4711).run()
+    ro = o.mc_run(n, 4711, mode=1)
+    kT = _kT(s)
+    n_ok = _compare_replay(res, ro, None, lambda de: 1e-5 * max(abs(de), kT))
+    assert n_ok == n
+    acc_by_type = np.bincount(ro["mvtyp"], weights=ro["acc"], minlength=4)
+    assert acc_by_type[2] > 0, "no chain-bead move accepted: the bonded terms' commit path was not exercised"
+    assert abs(res["etot"] - ro["etot"]) <= 1e-6 * max(e0, abs(ro["etot"]))
+    x, y, z = ff.get_coords()
+    assert max(np.max(np.abs(x - ref.x)), np.max(np.abs(y - ref.y)), np.max(np.abs(z - ref.z))) < 1e-9
+
+
 def _replay_system(kind):
     if kind == "cubic_full":
         s, _ = systems.water_box(seed=5, nside=5, L=15.53, nmol=125, tag=41)
@@ -354,6 +446,64 @@ def test_mc_replay_through_list_overflow():
     n_ok = _compare_replay(res, ro, None, lambda de: 1e-5 * max(abs(de), kT))
     assert n_ok == n
     assert ff.h.list_capacity() > m0, "no rebuild outgrew the capacity: the overflow path was not exercised"
+
+
+def _full_row_max(o):
+    cnt, rows = o.list_rows()
+    full = cnt.astype(np.int64).copy()        # full row = half row + the rows naming i
+    for r in rows:
+        np.add.at(full, r, 1)
+    return int(full.max())
+
+
+def test_mc_final_rebuild_overflow():
+    """The run ends on a scheduled rebuild that overflows the capacity:
+    with seed 777 on this box the oracle's rows first outgrow the first
+    build's largest row at the rebuild of step 300, and no chunk follows it.
+    The library must grow the list there, so that the energy asked for
+    afterwards sums complete rows and equals the oracle's."""
+    s, _ = systems.water_box(seed=3, nside=8, L=24.85)
+    o, ref = _pair(s)
+    o.update_list()
+    m0 = _full_row_max(o)
+    ff = mcx.FField(s)
+    assert ff.h.list_capacity(m0) == m0
+    n = 300
+    res = mcx.MC_metropolis(ff, nsteps=n, seed=777).run()
+    ro = o.mc_run(n, 777, mode=1)
+    kT = _kT(s)
+    assert _compare_replay(res, ro, None, lambda de: 1e-5 * max(abs(de), kT)) == n
+    o.update_list()
+    assert _full_row_max(o) > m0, "the last rebuild does not outgrow the capacity"
+    assert ff.h.list_capacity() > m0
+    _check_list(ff.h, o)
+    _check_energy(ff.getE10(), o.getE(), "after the final rebuild")
+
+
+def test_mc_frames_through_list_overflow():
+    """Frames every 100 steps with rebuilds every 50 and a rebuild that
+    overflows at step 300 (the chunks after it return untouched and the run
+    resumes there): every frame is written once, in step order, and the
+    chain equals the frameless one."""
+    s, _ = systems.water_box(seed=3, nside=8, L=24.85)
+    o, ref = _pair(s)
+    o.update_list()
+    m0 = _full_row_max(o)
+    e0 = o.getE()[0]
+    ff = mcx.FField(s)
+    ff.h.list_capacity(m0)
+    steps, etots = [], []
+    n = 1000
+    res = ff.h.mc_run(s.moves, n, 777, list_every=50, save_each=100,
+                      on_frame=lambda st, e, x, y, z: (steps.append(st), etots.append(e)))
+    assert steps == list(range(100, n + 1, 100)), steps
+    assert ff.h.list_capacity() > m0, "the overflow path was not exercised"
+    ro = o.mc_run(n, 777, mode=1)
+    kT = _kT(s)
+    assert _compare_replay(res, ro, None, lambda de: 1e-5 * max(abs(de), kT)) == n
+    run = e0 + np.cumsum(np.where(ro["acc"] == 1, ro["de"], 0.0))
+    for st, e in zip(steps, etots):
+        assert abs(e - run[st - 1]) <= 1e-6 * abs(e0), (st, e, run[st - 1])
 
 
 def test_gradient_vs_fp64_and_fd():

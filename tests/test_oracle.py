@@ -21,7 +21,6 @@ from oracle import oracle as O
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = json.load(open(os.path.join(HERE, "golden", "golden.json")))
-REF_TEST = "/root/reference/test"
 KCAL = 418.4
 
 
@@ -69,25 +68,14 @@ def test_reference_rint_is_truncation_after_half(oracle_lib):
 
 # ---------------------------------------------------------------- reference goldens
 
-def _ref_system(name):
-    g = GOLD[name]
-    if name.startswith("diala"):
-        cor, ff = os.path.join(HERE, "golden", "diala_vac", "ala2.cor"), os.path.join(HERE, "golden", "diala_vac", "ff.dat")
-    elif name == "argon_box":
-        cor, ff = os.path.join(HERE, "golden", "argon_box", "ar.cor"), os.path.join(HERE, "golden", "argon_box", "forfield.dat")
-    else:
-        d = os.path.join(REF_TEST, name)
-        if not os.path.isdir(d):
-            pytest.skip("reference test data not present")
-        cor = [os.path.join(d, f) for f in os.listdir(d) if f.endswith(".cor")][0]
-        ff = os.path.join(d, "forfield.dat")
-    assert hashlib.sha256(open(cor, "rb").read()).hexdigest() == g["cor_sha256"]
-    return systems.read_mdbas(cor, ff, g["box"], pbtype=g["pbtype"], cutmode=g["cutmode"]), g
+def _ref_system(name, tmp_path):
+    from tests.refdata import reference_system
+    return reference_system(name, tmp_path)
 
 
 @pytest.mark.parametrize("name", ["diala_vac", "diala_vac_switch", "argon_box", "nma_solvated", "trhbn_solvated"])
-def test_reference_test_systems_bit_exact(oracle_lib, name):
-    s, g = _ref_system(name)
+def test_reference_test_systems_bit_exact(oracle_lib, name, tmp_path):
+    s, g = _ref_system(name, tmp_path)
     o = O.Oracle(s)
     if s.cutmode == systems.CUT_SWITCH:
         o.update_list()
@@ -102,12 +90,17 @@ def test_reference_test_systems_bit_exact(oracle_lib, name):
 SMALL_BOX = dict(seed=21, nside=5, L=(90 / 0.0334) ** (1 / 3), nmol=90, temp=290.0, tag=300)
 
 
-@pytest.mark.parametrize("name", ["argon_fcc_seed1", "water512_seed3", "water4096_seed2", "water90_small_box"])
+@pytest.mark.parametrize("name", ["argon_fcc_seed1", "water512_seed3", "water4096_seed2", "water90_small_box",
+                                  "bonded_mix_seed7"])
 def test_synthetic_systems_bit_exact(oracle_lib, name):
+    """bonded_mix_seed7 carries every bonded kind the reference reads (Morse
+    bonds, Urey-Bradley, cosine and harmonic dihedrals and impropers) and
+    1-4 pairs, in an orthorhombic box, switched and all-pair."""
     gen = {"argon_fcc_seed1": lambda: systems.argon_fcc(seed=1),
            "water512_seed3": lambda: systems.water_box(seed=3, nside=8, L=24.85),
            "water4096_seed2": lambda: systems.water_box(seed=2),
-           "water90_small_box": lambda: systems.water_box(**SMALL_BOX)}[name]
+           "water90_small_box": lambda: systems.water_box(**SMALL_BOX),
+           "bonded_mix_seed7": lambda: systems.bonded_mix(seed=7)}[name]
     s, _ = gen()
     g = GOLD[name]
     o = O.Oracle(s)
@@ -138,16 +131,48 @@ def test_metropolis_run_matches_reference(oracle_lib, name):
     assert sha(np.stack([s.x, s.y, s.z], 1)) == g["final_xyz_sha256"]
 
 
-def test_moving_set_mode_equals_two_full_energies(oracle_lib):
-    s, _ = systems.water_box(seed=5, nside=6, L=18.64)
+def _small_bonded_mix(cutmode=systems.CUT_FULL):
+    return systems.bonded_mix(seed=7, nchain=3, nbead=12, box=(16.3, 17.9, 15.1), sites=(5, 5, 5), cutmode=cutmode)
+
+
+def _vacuum_dipeptide():
+    from mc_cpp_amd import deck
+    s = deck.read_deck(os.path.join(HERE, "golden", "decks", "diala_vac.xml")).system
+    s.pbtype = systems.PBC_NONE
+    return s
+
+
+@pytest.mark.parametrize("case", ["water_switch", "water_full_cubic", "bonded_full_orthorhombic",
+                                  "bonded_switch_orthorhombic", "dipeptide_full_vacuum"])
+def test_moving_set_mode_equals_two_full_energies(oracle_lib, case):
+    """The oracle's per-move dE (mode 1: the terms touching the moved set;
+    in all-pair mode the exclusions tested by membership) against the
+    reference's own construction, two full getE per move (mode 0,
+    src/MC_metropolis.cpp:94,122-123; all-pair exclusions by the sorted
+    pointer walk of src/FField_MDBAS.cpp:222-234): the same decisions and dE
+    within the rounding of two full FP64 totals."""
+    if case == "water_switch":
+        s, _ = systems.water_box(seed=5, nside=6, L=18.64)
+    elif case == "water_full_cubic":
+        s, _ = systems.water_box(seed=5, nside=5, L=15.53, nmol=125, tag=41)
+        s.cutmode = systems.CUT_FULL
+    elif case.startswith("bonded"):
+        s, _ = _small_bonded_mix(systems.CUT_FULL if "full" in case else systems.CUT_SWITCH)
+    else:
+        s = _vacuum_dipeptide()
     a = s.copy()
     oa, ob = O.Oracle(s), O.Oracle(a)
-    oa.update_list(); ob.update_list()
+    if s.cutmode == systems.CUT_SWITCH:
+        oa.update_list(); ob.update_list()
     ra = oa.mc_run(120, 99, mode=0)
     rb = ob.mc_run(120, 99, mode=1)
+    assert 0 < int(ra["acc"].sum()) < 120
     assert np.array_equal(ra["acc"], rb["acc"])
-    # two full FP64 totals differ from the moving-set sum by their own rounding (~1e-12 of E_tot)
-    assert np.max(np.abs(ra["de"] - rb["de"])) < 1e-11 * abs(ra["etot"])
+    # two full FP64 totals differ from the moving-set sum by their own
+    # rounding (~1e-12 of the larger of E_tot and the summed magnitudes)
+    scale = max(abs(ra["etot"]), np.max(np.abs(ra["de"])))
+    assert np.max(np.abs(ra["de"] - rb["de"])) < 1e-11 * scale
+    assert np.array_equal(s.x, a.x) and np.array_equal(s.y, a.y) and np.array_equal(s.z, a.z)
 
 
 def _brute_moved_energy(s, first, last):
@@ -344,6 +369,43 @@ def test_bonded_terms_at_reference_geometry(oracle_lib):
     s = systems.System(natom=n, box=[50.0] * 3, x=[-1.0, 0.0, 1.5, 2.5], y=[1.0, 0.0, 0.0, -1.0], z=[0.0] * 4)
     s.dihedrals = systems.make_terms("dihedrals", a=[0], b=[1], c=[2], d=[3], k=[10.0], type=[1], mult=[1.0], phi0=[0.0])
     assert abs(O.Oracle(s).getE()[8]) < 1e-12
+
+
+def _morse_pair(d, k_kcal=80.0, beta=1.8, r0=1.53):
+    s = systems.System(natom=2, box=[50.0] * 3, pbtype=systems.PBC_NONE, cutmode=systems.CUT_FULL,
+                       x=[0.0, d], y=[0.0, 0.0], z=[0.0, 0.0])
+    s.bonds = systems.make_terms("bonds", a=[0], b=[1], type=[1], k=[k_kcal * KCAL], r0=[r0], beta=[beta])
+    return O.Oracle(s).getE()
+
+
+def test_morse_bond_closed_forms(oracle_lib):
+    """BMORSE (src/FField_MDBAS.cpp:519-527), E(d) = D (e^{-2b(d-r0)} -
+    2 e^{-b(d-r0)}) + D: zero at r0 (its minimum), D where e^{-b(d-r0)} = 2
+    and in the dissociation limit, curvature 2 D b^2 at r0.  A harmonic
+    branch, a dropped factor or a wrong sign fails one of them."""
+    D, b, r0 = 80.0 * KCAL, 1.8, 1.53
+    e = _morse_pair(r0)
+    assert e[5] == 0.0 and e[0] == 0.0               # the 1-2 pair is excluded: the bond is all there is
+    assert _morse_pair(r0 - np.log(2.0) / b)[5] == pytest.approx(D, rel=1e-14)
+    assert _morse_pair(r0 + 40.0 / b)[5] == pytest.approx(D, rel=1e-14)
+    h = 1e-4
+    curv = (_morse_pair(r0 + h)[5] - 2 * _morse_pair(r0)[5] + _morse_pair(r0 - h)[5]) / h ** 2
+    assert curv == pytest.approx(2 * D * b * b, rel=1e-6)
+    # asymmetry: stretching costs less than compressing by the same amount
+    assert _morse_pair(r0 + 0.2)[5] < _morse_pair(r0 - 0.2)[5]
+    x = 0.37
+    assert _morse_pair(r0 + x)[5] == pytest.approx(D * (1 - np.exp(-b * x)) ** 2, rel=1e-13)
+
+
+def test_urey_bradley_closed_form(oracle_lib):
+    """computeEub (src/FField_MDBAS.cpp:576-600): k (d - r0)^2 / 2 over the
+    1-3 distance, zero at r0."""
+    k = 30.0 * KCAL
+    for d, want in ((2.52, 0.0), (2.9, 0.5 * k * 0.38 ** 2), (2.1, 0.5 * k * 0.42 ** 2)):
+        s = systems.System(natom=3, box=[50.0] * 3, pbtype=systems.PBC_NONE, cutmode=systems.CUT_FULL,
+                           x=[0.0, 0.5 * d, d], y=[0.0, 0.9, 0.0], z=[0.0, 0.0, 0.0])
+        s.ub = systems.make_terms("ub", a=[0], b=[2], k=[k], r0=[2.52])
+        assert O.Oracle(s).getE()[7] == pytest.approx(want, rel=1e-12, abs=1e-9)
 
 
 def test_rigid_moves_preserve_molecules(oracle_lib):
