@@ -129,6 +129,18 @@ int mcx_list_capacity(mcx_ctx *h, int set, int *current);
  * {tot, pot, kin, elec, vdw, bond, ang, ub, dihe, impr} in iu. */
 int mcx_energy(mcx_ctx *h, double ener[10]);
 
+/* mcx_energy with no device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Partial sums and reduced components live in one static 4 MiB device
+ * array per GPU, held until the process ends; all-pair mode supports at
+ * most 46,272 atoms, switched mode 1,048,544 (MCX_ERR_ARG beyond).
+ * Switched mode needs a current pair list (MCX_ERR_STATE otherwise: call
+ * mcx_update_list, which may allocate, beforehand).  Same result bits as
+ * mcx_energy.  Calls on one GPU share the array: calls that overlap (other
+ * handles, streams or threads on that GPU) race and must be serialised. */
+int mcx_energy_noalloc(mcx_ctx *h, double ener[10]);
+
 /* Energy change of moving atoms [first,last] to the given coordinates
  * (arrays of last-first+1 entries), against the current list; equal to
  * getE(new) - getE(old) as MC_metropolis::run forms it

@@ -592,7 +592,10 @@ int mcx_create(int natom, int device, mcx_ctx **out)
     h->n = natom;
     h->device = device;
     int rc = guarded(h.get(), [&] {
+        HIPCHK(hipSetDevice(device));
         HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        if (noalloc_init_device(device) != MCX_OK)
+            throw mcx_error(MCX_ERR_HIP, "address of the device's no-allocation scratch array not found");
         h->q.assign(natom, 0); h->eps.assign(natom, 0); h->sig.assign(natom, 0);
         h->eps14.assign(natom, 0); h->sig14.assign(natom, 0); h->mass.assign(natom, 1.0);
         h->frozen.assign(natom, 0);

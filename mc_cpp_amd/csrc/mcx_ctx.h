@@ -171,6 +171,10 @@ struct mcx_ctx {
 };
 
 
+// noalloc.hip: looks up the device's copy of the static scratch array of
+// mcx_energy_noalloc once, with that device current (called by mcx_create).
+int noalloc_init_device(int device);
+
 // host helpers (mcx_api.cpp)
 namespace mcxi {
 long long build_list(mcx_ctx *h);

@@ -152,6 +152,9 @@ struct mcx_error {
 void launch_full_energy(const DevSys &s, const int2 *tiles, int ntiles, int tile, double *partial,
                         hipStream_t st);
 void launch_switch_energy(const DevSys &s, double *partial, int nblocks, hipStream_t st);
+// k_full_energy with the tile of each block computed instead of read from a
+// tile list: ceil(n/64)*(ceil(n/64)+1)/2 blocks, two partials each.
+void launch_full_energy_listless(const DevSys &s, double *partial, hipStream_t st);
 void launch_reduce(const double *partial, int nblocks, int ncomp, double *out, hipStream_t st);
 int switch_energy_blocks(int n);
 // list.hip

@@ -118,11 +118,27 @@ __device__ __forceinline__ void full_tile(const DevSys &s, int i, int j0, int j1
         tile_pair_acc<PB, DIAG>(s, i, j, pi, ipos[j], tq[j], xi, yi, zi, si, accC, accV);
 }
 
-template <int PB>
-__global__ __launch_bounds__(kTile) void k_full_energy(DevSys s, const int2 *__restrict__ tiles,
-                                                        double *__restrict__ partial)
+// Tile b of the triangular enumeration that the tile list holds: row I
+// has the tiles (I, I), (I, I+1), ..., (I, nt-1) and starts at
+// S(I) = I*nt - I*(I-1)/2.
+__device__ __forceinline__ int2 tri_tile(int b, int nt)
 {
-    const int2 tl = tiles[blockIdx.x];
+    auto S = [nt](long long I) { return I * nt - I * (I - 1) / 2; };
+    const double a = 2.0 * nt + 1.0;
+    int I = (int)(0.5 * (a - sqrt(fmax(a * a - 8.0 * b, 0.0))));
+    I = I < 0 ? 0 : (I >= nt ? nt - 1 : I);
+    while (I > 0 && S(I) > b) I--;
+    while (I + 1 < nt && S(I + 1) <= b) I++;
+    return make_int2(I, I + (int)(b - S(I)));
+}
+
+// LIST: tile b read from `tiles`; otherwise computed (tri_tile over nt
+// tile rows), for callers that keep no tile list.
+template <int PB, bool LIST = true>
+__global__ __launch_bounds__(kTile) void k_full_energy(DevSys s, const int2 *__restrict__ tiles,
+                                                        double *__restrict__ partial, int nt = 0)
+{
+    const int2 tl = LIST ? tiles[blockIdx.x] : tri_tile(blockIdx.x, nt);
     const int i = tl.x * kTile + threadIdx.x;
     const bool valid = i < s.n;
     const int ii = valid ? i : 0;
@@ -230,6 +246,19 @@ void launch_full_energy(const DevSys &s, const int2 *tiles, int ntiles, int tile
     if (pb == 1) hipLaunchKernelGGL(k_full_energy<1>, dim3(ntiles), dim3(kTile), 0, st, s, tiles, partial);
     else if (pb == 2) hipLaunchKernelGGL(k_full_energy<2>, dim3(ntiles), dim3(kTile), 0, st, s, tiles, partial);
     else hipLaunchKernelGGL(k_full_energy<0>, dim3(ntiles), dim3(kTile), 0, st, s, tiles, partial);
+}
+
+void launch_full_energy_listless(const DevSys &s, double *partial, hipStream_t st)
+{
+    const int nt = (s.n + kTile - 1) / kTile;
+    const int ntiles = nt * (nt + 1) / 2;
+    const int pb = s.pbtype == 0 ? 0 : (s.pbtype == 1 ? 1 : 2);
+    if (pb == 1)
+        hipLaunchKernelGGL((k_full_energy<1, false>), dim3(ntiles), dim3(kTile), 0, st, s, nullptr, partial, nt);
+    else if (pb == 2)
+        hipLaunchKernelGGL((k_full_energy<2, false>), dim3(ntiles), dim3(kTile), 0, st, s, nullptr, partial, nt);
+    else
+        hipLaunchKernelGGL((k_full_energy<0, false>), dim3(ntiles), dim3(kTile), 0, st, s, nullptr, partial, nt);
 }
 
 int switch_energy_blocks(int n)

@@ -88,6 +88,7 @@ SIGNATURES = {
     "mcx_get_list": ([C.c_void_p, _ip, _ip, C.c_longlong], C.c_int),
     "mcx_list_capacity": ([C.c_void_p, C.c_int, _ip], C.c_int),
     "mcx_energy": ([C.c_void_p, _dp], C.c_int),
+    "mcx_energy_noalloc": ([C.c_void_p, _dp], C.c_int),
     "mcx_delta_energy": ([C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp], C.c_int),
     "mcx_mc_run": ([C.c_void_p, C.POINTER(MoveTypeC), C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int,
                     C.POINTER(MCStatsC), C.POINTER(C.c_ubyte), _dp, _ip, _ip], C.c_int),
@@ -254,6 +255,13 @@ class Handle:
     def energy(self):
         e = np.zeros(10)
         self._chk(self.L.mcx_energy(self.h, _p(e, C.c_double)))
+        return e
+
+    def energy_noalloc(self):
+        """mcx_energy_noalloc: the energy with no device allocation (a
+        current pair list is required in switched mode)."""
+        e = np.zeros(10)
+        self._chk(self.L.mcx_energy_noalloc(self.h, _p(e, C.c_double)))
         return e
 
     def delta_energy(self, first, last, nx, ny, nz):
