@@ -828,6 +828,415 @@ __global__ __launch_bounds__(kMCT) void k_mc_spec(DevSys s0, DevTerms t, DevMove
         for (int k = tid; k < kMTN; k += kMCT) arg.mt_state[k] = mt[k];
 }
 
+// ---------------------------------------------------------------------------
+// Speculative rounds over several workgroups.  The read-ahead argument above
+// extends to any depth: as long as the moves before it are rejected, the
+// proposal g moves ahead starts at a draw position known in advance (each
+// rejected move used its proposal draws and one acceptance draw) and sees
+// the coordinates as they are now.  k_mc_wide runs nwg workgroups (one per
+// CU, all resident: a cooperative launch).  In a round every workgroup
+// parses the same stretch of the draw stream into up to nwg proposals;
+// workgroup g draws and loads proposal g and evaluates its dE with the pair
+// and bonded code of k_mc_chunk; the dE values meet on a board in global
+// memory behind a grid-wide barrier; then every workgroup resolves the round
+// identically in chain order up to and including the first accepted move,
+// and writes that move's new coordinates itself (the same values from every
+// workgroup), so that its own view is current without a second barrier.
+// The next round starts after the last resolved move: its acceptance draw
+// consumed, unless it was accepted with dE <= DBL_EPSILON (no draw).  A
+// round never extends past a dmax adaptation step.  The chain is k_mc_chunk's
+// bit for bit: the same proposals, the same dE (same code, same reduction
+// order), the same decisions.
+//
+// Draws: each workgroup generates the stream (MT19937 state in LDS, or
+// Philox) into a ring of kWideRing draws ahead of the chain's position.
+
+struct RingDraws {
+    const double *ring;
+    long long idx, end;
+    bool ok;
+    __device__ __forceinline__ double next(int)
+    {
+        double u = 0.5;                   // past the generated draws: ends a rejection loop; ok cleared
+        if (idx < end) u = ring[idx & (kWideRing - 1)];
+        else ok = false;
+        idx++;
+        return u;
+    }
+};
+
+// The draws of one proposal without its loads: exactly what pick_instance
+// and pick_displacement consume for a TRN or ROT move.
+template <class D>
+__device__ __forceinline__ void skip_proposal(const DevMoves &mv, D &dr, int lane)
+{
+    const int tp = (int)((double)mv.ntypes * dr.next(lane));
+    (void)dr.next(lane);
+    const int type = mv.type[tp < mv.ntypes ? tp : mv.ntypes - 1];
+    double r0, r1, r2;
+    do {
+        r0 = dr.next(lane) * 2.0 + (-1.0);
+        r1 = dr.next(lane) * 2.0 + (-1.0);
+        r2 = dr.next(lane) * 2.0 + (-1.0);
+    } while ((r0 * r0 + r1 * r1 + r2 * r2) > 1.0);
+    if (type == MCX_MV_ROT_DEV) (void)dr.next(lane);
+}
+
+struct WideLDS {
+    MoveLDS m;                         // this workgroup's proposal
+    double ring[kWideRing];
+    int endq[kWideRing];               // position of the acceptance draw of the proposal starting at
+                                       // each ring position, relative to it (-1: not yet known)
+    long long start[kMaxWide];         // draw position of each proposal of the round
+    long long accpos[kMaxWide];        // position of its acceptance draw
+    double accu[kMaxWide];             // its acceptance draw
+    int nprop;                         // proposals of the round (0: the chunk is over)
+    int fail;
+};
+
+// Grid-wide barrier of a round: every workgroup's writes before it are
+// visible to every workgroup after it (agent-scope release before the
+// arrival, acquire after the wait).  A wait gives up after spin_ticks of
+// the constant-rate clock, or as soon as another workgroup gave up, and
+// then reports failure: the kernel ends and the host reports the error.
+__device__ __forceinline__ bool round_barrier(const WideArgs &w, unsigned target, int &s_fail)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(w.sync, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const long long t0 = (long long)wall_clock64();
+        int fail = 0;
+        while (__hip_atomic_load(w.sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if ((long long)wall_clock64() - t0 > w.spin_ticks ||
+                __hip_atomic_load(w.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                fail = 1;
+                __hip_atomic_store(w.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        s_fail = fail;
+    }
+    __syncthreads();
+    return s_fail == 0;
+}
+
+// First step >= st0 (1-based) after which dmax is adapted, or INT_MAX.
+__device__ __forceinline__ long long next_adapt(const DevMoves &mv, long long st0)
+{
+    long long best = LLONG_MAX;
+    for (int k = 0; k < mv.ntypes; k++) {
+        const long long e = mv.each[k];
+        if (e <= 0) continue;
+        const long long s = ((st0 + e - 1) / e) * e;
+        if (s < best) best = s;
+    }
+    return best;
+}
+
+template <int PB, int CUT>
+__global__ __launch_bounds__(kMCT) void k_mc_wide(DevSys s0, DevTerms t, DevMoves mv, MCRunArgs arg, WideArgs w)
+{
+    const DevSys s = specialise<PB, CUT>(s0);
+    if (arg.bail && (s.overflow[0] != 0 || arg.bail[0] != INT_MAX)) {
+        // the list this chunk would use is incomplete: leave every state
+        // untouched; the host grows the list and runs again from here
+        if (threadIdx.x == 0 && blockIdx.x == 0) atomicMin(arg.bail, arg.step0);
+        return;
+    }
+    __shared__ WideLDS g;
+    __shared__ uint32_t mt[kMTN];
+    __shared__ double dmax[kMaxMoveTypes];
+    __shared__ int acc_tmp[kMaxMoveTypes];
+    __shared__ long long n_trial[kMaxMoveTypes], n_acc[kMaxMoveTypes];
+    __shared__ double etot;
+    __shared__ int s_stop, s_done;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int me = blockIdx.x;
+    const bool w0 = __builtin_amdgcn_readfirstlane(tid) < 64;   // wave-uniform
+    MoveLDS &m = g.m;
+    if (arg.rng_mode == 1)
+        for (int k = tid; k < kMTN; k += kMCT) mt[k] = arg.mt_state[k];
+    if (tid == 0) { s_stop = 0; s_done = 0; g.fail = 0; }
+    __syncthreads();
+    WaveDraws gen;
+    long long pos = 0, filled = 0;
+    long long eq_hi = 0;           // endq holds the ends of proposals starting in [pos, eq_hi)
+    long long g_start_l = 0;       // lane q: start of proposal q of the round
+    // wavefront 0: 32 more draws into the ring
+    auto gen32 = [&]() {
+        gen.refill(lane);
+        if (lane < 32) g.ring[(filled + lane) & (kWideRing - 1)] = gen.v;
+        filled += 32;
+        wave_sync();
+    };
+    // generate up to `upto` while the ring has room
+    auto top_up = [&](long long upto) {
+        while (filled < upto && filled - pos + 32 <= kWideRing) gen32();
+    };
+    // where the proposals starting at q0 .. q0+63 end (their acceptance
+    // draw, relative to the start; -1: past the generated draws)
+    auto parse_chunk = [&](long long q0) {
+        const long long q = q0 + lane;
+        RingDraws d{g.ring, q, filled, true};
+        skip_proposal(mv, d, lane);
+        g.endq[q & (kWideRing - 1)] = (d.ok && d.idx < filled) ? (int)(d.idx - q) : -1;
+        wave_sync();
+    };
+    // during a round's dE: fill the ring, parse up to four chunks ahead
+    auto parse_ahead = [&]() {
+        top_up(pos + kWideRing);
+        if (eq_hi < pos) eq_hi = pos;
+        for (int c = 0; c < 4 && eq_hi + 64 - pos <= kWideRing; c++) {
+            parse_chunk(eq_hi);
+            eq_hi += 64;
+        }
+    };
+    if (w0) {
+        gen.replay = arg.rng_mode == 1;
+        gen.mt = mt;
+        gen.key = make_uint2((uint32_t)arg.seed, (uint32_t)(arg.seed >> 32));
+        gen.chain = arg.stream;
+        gen.load(arg.mt_state, lane);
+        const uint32_t *st = arg.mt_state;
+        const uint32_t flag = st[kRngWords - 1];
+        if (flag == 2u) {
+            // drawn-but-unused draws of an earlier wide chunk
+            const int cnt = (int)st[kRngWords];
+            for (int k = lane; k < cnt; k += 64)
+                g.ring[k] = __hiloint2double((int)st[kRngWords + 2 + 2 * k], (int)st[kRngWords + 1 + 2 * k]);
+            filled = cnt;
+        } else {
+            // the rest of the current batch, then the batch read ahead
+            const int c = gen.cur;
+            if (lane >= c && lane < 32) g.ring[lane - c] = gen.v;
+            filled = 32 - c;
+            if (flag == 1u) {
+                if (lane < 32)
+                    g.ring[filled + lane] = __hiloint2double((int)st[kRngAhead + 2 * lane + 1], (int)st[kRngAhead + 2 * lane]);
+                filled += 32;
+            }
+        }
+        if (lane < kMaxMoveTypes && lane < mv.ntypes) {
+            dmax[lane] = arg.dmax[lane]; acc_tmp[lane] = arg.acc_tmp[lane];
+            n_trial[lane] = arg.trials[lane]; n_acc[lane] = arg.accepted[lane];
+        }
+        if (lane == 0) etot = arg.etot[0];
+        wave_sync();
+    }
+    __syncthreads();
+
+    int done = 0;
+    unsigned round = 0;
+    bool ok = true;
+    for (;;) {
+        // ---- the round's proposals (wavefront 0 of every workgroup, the
+        // same in all): draw positions and acceptance draws, by following
+        // the chain of proposals through endq (where each proposal ends;
+        // mostly parsed ahead during the previous round's dE).  Where a
+        // round stops depends on the draw stream alone, never on how much
+        // a workgroup generated or parsed ahead.
+        if (w0) {
+            int np = 0;
+            const int left = arg.nsteps - done;
+            if (eq_hi < pos) eq_hi = pos;
+            if (left > 0 && !s_stop) {
+                long long wmax = min(left, w.nwg);
+                const long long na = next_adapt(mv, (long long)arg.step0 + done + 1);
+                if (na != LLONG_MAX) wmax = min(wmax, na - ((long long)arg.step0 + done));
+                long long p = pos;
+                while (np < wmax && p + 64 - pos <= kWideRing) {
+                    if (p >= eq_hi) {
+                        top_up(eq_hi + 192);
+                        parse_chunk(eq_hi);
+                        eq_hi += 64;
+                        continue;
+                    }
+                    int e = g.endq[p & (kWideRing - 1)];
+                    if (e < 0) {
+                        // the proposal at p ran past the draws generated when
+                        // it was parsed: generate more (while the ring has
+                        // room) and parse again; with a full ring the result
+                        // is final
+                        const bool full = filled - pos + 32 > kWideRing;
+                        if (!full) gen32();
+                        parse_chunk(p);
+                        if (p + 64 > eq_hi) eq_hi = p + 64;
+                        if (full && g.endq[p & (kWideRing - 1)] < 0) break;
+                        continue;
+                    }
+                    if (lane == np) g_start_l = p;
+                    p += e + 1;
+                    np++;
+                }
+            }
+            // a chunk that still has moves always gets a proposal, unless one
+            // proposal needs more draws than the ring holds (then stop and
+            // report: the host raises an error)
+            if (np == 0 && left > 0 && !s_stop && lane == 0)
+                __hip_atomic_store(w.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // lane q < np: proposal q's start, acceptance-draw position and draw
+            if (lane < np) {
+                const long long ap = g_start_l + g.endq[g_start_l & (kWideRing - 1)];
+                g.start[lane] = g_start_l;
+                g.accpos[lane] = ap;
+                g.accu[lane] = g.ring[ap & (kWideRing - 1)];
+            }
+            if (lane == 0) g.nprop = np;
+            wave_sync();
+            // this workgroup's proposal: draw and load it, form the trial
+            if (me < np) {
+                RingDraws d{g.ring, g.start[me], filled, true};
+                pick_load(s, mv, d, dmax, m, lane, arg.ipos_ref, arg.half_skin2);
+            }
+        }
+        __syncthreads();
+        const int np = g.nprop;
+        if (np == 0) break;
+        // ---- dE of this workgroup's proposal; meanwhile wavefront 0 fills
+        // the draw ring and parses ahead for the next round
+        if (me < np) {
+            double dec = 0.0, dev = 0.0, deb = 0.0;
+            double fnew[3] = {0.0, 0.0, 0.0};
+            if (w0) {
+                delta_terms(s, mv, m, dec, dev, deb);
+                parse_ahead();
+            } else {
+                delta_pairs(s, m, false, dec, dev, fnew);
+            }
+            delta_finish(m, false, dec, dev, deb, fnew);
+            if (w0) {
+                if (lane == 0) {
+                    w.bde[me] = m.de;
+                    w.bmeta[me] = make_int4(m.acc, m.first, m.nm, m.inst);
+                }
+                for (int a = lane; a < m.nm; a += 64) {
+                    double *b = w.bxyz + ((size_t)me * kMaxMoveAtoms + a) * 3;
+                    b[0] = m.tx[a]; b[1] = m.ty[a]; b[2] = m.tz[a];
+                    w.bip[(size_t)me * kMaxMoveAtoms + a] =
+                        mcx_fixed4(s, m.tx[a], m.ty[a], m.tz[a], __int_as_float(m.ip[a].w));
+                }
+            }
+        } else if (w0) {
+            parse_ahead();
+        }
+        round++;
+        if (!round_barrier(w, round * (unsigned)w.nwg, g.fail)) { ok = false; break; }
+        // ---- resolve in chain order (wavefront 0, the same in every
+        // workgroup).  Lane q takes proposal q's dE and acceptance test and
+        // also reads its first four trial positions, so that the commit of
+        // a move of up to four atoms needs no further round trip.
+        if (w0) {
+            const double de_l = lane < np ? w.bde[lane] : 0.0;
+            const int4 meta_l = lane < np ? w.bmeta[lane] : make_int4(0, 0, 0, 0);
+            const int pq = lane >> 2, pa = lane & 3;
+            double cx = 0.0, cy = 0.0, cz = 0.0;
+            int4 cip = make_int4(0, 0, 0, 0);
+            if (pq < np) {
+                const double *b = w.bxyz + ((size_t)pq * kMaxMoveAtoms + pa) * 3;
+                cx = b[0]; cy = b[1]; cz = b[2];
+                cip = w.bip[(size_t)pq * kMaxMoveAtoms + pa];
+            }
+            // MC_metropolis::apply_criterion (src/MC_metropolis.cpp:175-204)
+            const bool free_acc = de_l <= DBL_EPSILON;
+            const bool acc_l = lane < np && (free_acc || g.accu[lane < kMaxWide ? lane : 0] < exp(-arg.beta * de_l));
+            const uint64_t accm = __ballot(acc_l);
+            const int acc_at = accm ? __builtin_ctzll(accm) : -1;
+            const int R = acc_at >= 0 ? acc_at + 1 : np;
+            if (lane == 0) {
+                for (int q = 0; q < R; q++) {
+                    const double de = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(de_l), q),
+                                                       __builtin_amdgcn_readlane(__double2loint(de_l), q));
+                    const int tp = __builtin_amdgcn_readlane(meta_l.x, q);
+                    const int acc = q == acc_at;
+                    n_trial[tp]++;
+                    if (acc) { etot += de; n_acc[tp]++; }
+                    adapt_dmax(mv, (long long)arg.step0 + done + q + 1, tp, acc, dmax, acc_tmp);
+                }
+            }
+            if (me < R && lane == 0) {
+                // this workgroup's own move: its per-step records
+                const int o = arg.step0 + done + me;
+                if (arg.acc_out) arg.acc_out[o] = (unsigned char)(me == acc_at);
+                if (arg.de_out) arg.de_out[o] = m.de;
+                if (arg.mvtyp_out) arg.mvtyp_out[o] = m.acc;
+                if (arg.mvinst_out) arg.mvinst_out[o] = m.inst;
+                if (arg.lnq_out) arg.lnq_out[o] = 0.0;
+                if (arg.disp_out) {
+                    const bool tr = m.type != MCX_MV_ROT_DEV;
+                    for (int c = 0; c < 3; c++) arg.disp_out[3 * o + c] = tr ? m.r[c] : 0.0;
+                }
+            }
+            // the chain's next draw: after the last resolved proposal's
+            // acceptance draw, or at it if that move was accepted with
+            // dE <= DBL_EPSILON (apply_criterion then draws nothing)
+            const bool last_free = __builtin_amdgcn_readlane((int)free_acc, R - 1) != 0;
+            pos = g.accpos[R - 1] + ((acc_at >= 0 && last_free) ? 0 : 1);
+            // the accepted move's new coordinates, written by every
+            // workgroup (the same values)
+            bool far = false;
+            if (acc_at >= 0) {
+                const int first = __builtin_amdgcn_readlane(meta_l.y, acc_at);
+                const int nm = __builtin_amdgcn_readlane(meta_l.z, acc_at);
+                // positions read above: the first four atoms of proposals 0-15
+                const int a0 = acc_at < 16 ? 4 : 0;
+                if (acc_at < 16 && pq == acc_at && pa < nm) {
+                    const int i = first + pa;
+                    s.x[i] = cx; s.y[i] = cy; s.z[i] = cz;
+                    s.ipos[i] = cip;
+                    far = arg.ipos_ref && beyond_half_skin(s, cip, arg.ipos_ref[i], arg.half_skin2);
+                }
+                for (int a = a0 + lane; a < nm; a += 64) {
+                    const int i = first + a;
+                    const double *b = w.bxyz + ((size_t)acc_at * kMaxMoveAtoms + a) * 3;
+                    const int4 np4 = w.bip[(size_t)acc_at * kMaxMoveAtoms + a];
+                    s.x[i] = b[0]; s.y[i] = b[1]; s.z[i] = b[2];
+                    s.ipos[i] = np4;
+                    far = far || (arg.ipos_ref && beyond_half_skin(s, np4, arg.ipos_ref[i], arg.half_skin2));
+                }
+            }
+            done += R;
+            if (__ballot(far) != 0ull && lane == 0) s_stop = 1;
+            if (lane == 0) s_done = done;
+            wave_sync();
+        }
+        __syncthreads();
+        done = s_done;
+    }
+    if (!ok) return;
+    if (arg.stop_at && tid == 0 && me == 0) {
+        arg.stop_at[0] = done;
+        arg.stop_at[1] = s_stop;
+    }
+    if (me != 0) return;
+    __syncthreads();
+    if (w0) {
+        if (lane < mv.ntypes) {
+            arg.dmax[lane] = dmax[lane]; arg.acc_tmp[lane] = acc_tmp[lane];
+            arg.trials[lane] = n_trial[lane]; arg.accepted[lane] = n_acc[lane];
+        }
+        if (lane == 0) arg.etot[0] = etot;
+        // generator state, then the drawn-but-unused draws [pos, filled)
+        gen.save(arg.mt_state, lane);
+        uint32_t *st = arg.mt_state;
+        const int cnt = (int)(filled - pos);
+        for (int k = lane; k < cnt; k += 64) {
+            const double u = g.ring[(pos + k) & (kWideRing - 1)];
+            st[kRngWords + 1 + 2 * k] = (uint32_t)__double2loint(u);
+            st[kRngWords + 2 + 2 * k] = (uint32_t)__double2hiint(u);
+        }
+        if (lane == 0) {
+            st[kRngWords] = (uint32_t)cnt;
+            st[kRngWords - 1] = 2u;
+        }
+    }
+    if (arg.rng_mode == 1)
+        for (int k = tid; k < kMTN; k += kMCT) arg.mt_state[k] = mt[k];
+}
+
 // one-shot dE for caller-supplied trial coordinates
 __global__ __launch_bounds__(kMCT) void k_delta(DevSys s, DevTerms t, DevMoves mv, int first, int last,
                                                 const double *__restrict__ nxyz, double *out)
@@ -866,6 +1275,37 @@ void launch_delta_energy(const DevSys &s, const DevTerms &t, const DevMoves &mv,
                          const double *newxyz, double *de, hipStream_t st)
 {
     hipLaunchKernelGGL(k_delta, dim3(1), dim3(kMCT), 0, st, s, t, mv, first, last, newxyz, de);
+}
+
+hipError_t launch_mc_wide(const DevSys &s, const DevTerms &t, const DevMoves &mv, const MCRunArgs &a,
+                          const WideArgs &w, hipStream_t st)
+{
+    DevSys s_ = s;
+    DevTerms t_ = t;
+    DevMoves mv_ = mv;
+    MCRunArgs a_ = a;
+    WideArgs w_ = w;
+    void *args[] = {&s_, &t_, &mv_, &a_, &w_};
+    const void *f;
+    if (s.pbtype == MCX_PBC_CUBIC && s.cutmode == MCX_CUT_SWITCH)
+        f = reinterpret_cast<const void *>(&k_mc_wide<MCX_PBC_CUBIC, MCX_CUT_SWITCH>);
+    else if (s.pbtype == MCX_PBC_CUBIC && s.cutmode == MCX_CUT_FULL)
+        f = reinterpret_cast<const void *>(&k_mc_wide<MCX_PBC_CUBIC, MCX_CUT_FULL>);
+    else
+        f = reinterpret_cast<const void *>(&k_mc_wide<-1, -1>);
+    return hipLaunchCooperativeKernel(f, dim3(w.nwg), dim3(kMCT), args, 0, st);
+}
+
+int mc_wide_max_workgroups(int device)
+{
+    // one workgroup of kMCT threads per CU at most (LDS and registers), so
+    // the co-resident limit is the occupancy times the CU count
+    int ncu = 0, per = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void *>(&k_mc_wide<-1, -1>),
+                                                     kMCT, 0) != hipSuccess)
+        return 0;
+    return ncu * per;
 }
 
 void launch_mc_chunk(const DevSys &s, const DevTerms &t, const DevMoves &mv, const MCRunArgs &a, hipStream_t st)

@@ -928,7 +928,7 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
         std::vector<double> dm(ntypes);
         for (int t = 0; t < ntypes; t++) dm[t] = types[t].dmax;
         dmax.upload(dm.data(), ntypes, h->stream);
-        std::vector<uint32_t> st(kRngWords, 0);
+        std::vector<uint32_t> st(kRngWordsWide, 0);
         mt_seed(st.data(), seed);
         st[625] = 0; st[626] = 0;
         st[627] = 32;     // empty draw batch: the first draw refills it
@@ -955,6 +955,55 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
         a.spec = !(sp && sp[0] == '0');
         for (int k = 0; k < ntypes; k++)
             if (types[k].type == MCX_MV_FBIAS) a.spec = 0;
+        // speculative rounds over several workgroups (k_mc_wide) for the
+        // same move sets; MCX_WIDE=<n> sets the workgroup count, 0 or 1
+        // keeps the single-workgroup read-ahead kernel
+        int nwg = 0;
+        if (a.spec) {
+            const char *wv = std::getenv("MCX_WIDE");
+            const int want = wv ? std::atoi(wv) : kDefaultWide;
+            if (want > 1) nwg = std::min(std::min(want, kMaxWide), mc_wide_max_workgroups(h->device));
+            if (nwg < 2) nwg = 0;
+        }
+        DBuf<unsigned> sync_d;
+        DBuf<int> werr_d;
+        DBuf<double> bde_d, bxyz_d;
+        DBuf<int4> bmeta_d, bip_d;
+        cleanup.f.push_back([&] { sync_d.release(); werr_d.release(); bde_d.release(); bxyz_d.release();
+                                  bmeta_d.release(); bip_d.release(); });
+        WideArgs wa{};
+        if (nwg) {
+            sync_d.alloc(1);
+            const int zero = 0;
+            werr_d.upload(&zero, 1, h->stream);
+            bde_d.alloc(kMaxWide);
+            bmeta_d.alloc(kMaxWide);
+            bxyz_d.alloc((size_t)kMaxWide * kMaxMoveAtoms * 3);
+            bip_d.alloc((size_t)kMaxWide * kMaxMoveAtoms);
+            int rate_khz = 0;
+            HIPCHK(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, h->device));
+            wa.nwg = nwg;
+            wa.sync = sync_d.p;
+            wa.err = werr_d.p;
+            wa.spin_ticks = (long long)std::max(rate_khz, 1000) * 500;    // half a second
+            wa.bde = bde_d.p; wa.bmeta = bmeta_d.p; wa.bxyz = bxyz_d.p; wa.bip = bip_d.p;
+        }
+        auto launch_chunk = [&]() {
+            if (nwg) {
+                HIPCHK(hipMemsetAsync(sync_d.p, 0, sizeof(unsigned), h->stream));
+                HIPCHK(launch_mc_wide(h->sys(), h->dterms(), mv, a, wa, h->stream));
+            } else {
+                launch_mc_chunk(h->sys(), h->dterms(), mv, a, h->stream);
+            }
+        };
+        auto check_wide = [&]() {
+            if (!nwg) return;
+            int e = 0;
+            HIPCHK(hipMemcpyAsync(&e, werr_d.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+            if (e == 2) throw mcx_error(MCX_ERR_HIP, "a proposal needed more draws than the speculative kernel holds");
+            if (e) throw mcx_error(MCX_ERR_HIP, "a speculative round barrier gave up waiting (MCX_WIDE=0 avoids it)");
+        };
         a.etot = etot.p; a.mt_state = mts.p; a.dmax = dmax.p; a.trials = trials.p; a.accepted = accepted.p;
         a.acc_tmp = acc_tmp.p;
         a.acc_out = acc_out ? acc_d.p : nullptr;
@@ -994,8 +1043,9 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
             a.nsteps = std::min(kAutoChunk, nsteps - done);
             if (save) a.nsteps = std::min(a.nsteps, save - done % save);
             a.step0 = done;
-            launch_mc_chunk(h->sys(), h->dterms(), mv, a, h->stream);
+            launch_chunk();
             HIPCHK(hipGetLastError());
+            check_wide();
             int st[2] = {0, 0};
             HIPCHK(hipMemcpyAsync(st, stop_d.p, sizeof(st), hipMemcpyDeviceToHost, h->stream));
             HIPCHK(hipStreamSynchronize(h->stream));
@@ -1023,6 +1073,7 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
         }
         int since_check = 0;
         auto check_bail = [&]() {
+            check_wide();
             if (!a.bail) return;
             int b = kNone;
             HIPCHK(hipMemcpyAsync(&b, bail_d.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
@@ -1041,7 +1092,7 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
             if (chunk > nsteps - done) chunk = nsteps - done;
             a.nsteps = chunk;
             a.step0 = done;
-            launch_mc_chunk(h->sys(), h->dterms(), mv, a, h->stream);
+            launch_chunk();
             HIPCHK(hipGetLastError());
             done += chunk;
             if (rebuilds && done % list_every == 0) {
@@ -1069,6 +1120,7 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
             }
         }
         HIPCHK(hipStreamSynchronize(h->stream));
+        check_wide();
         const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (stats) {
             std::memset(stats, 0, sizeof(*stats));

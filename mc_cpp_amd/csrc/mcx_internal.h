@@ -206,6 +206,31 @@ struct MCRunArgs {
 };
 void launch_mc_chunk(const DevSys &s, const DevTerms &t, const DevMoves &mv, const MCRunArgs &a,
                      hipStream_t st);
+// Speculative rounds over several workgroups (k_mc_wide): workgroup g of a
+// round evaluates the proposal g moves ahead on the assumption that the
+// moves before it are rejected; the rounds resolve in chain order.
+constexpr int kMaxWide = 32;        // workgroups of one launch
+constexpr int kDefaultWide = 8;     // workgroups unless MCX_WIDE says otherwise
+constexpr int kWideRing = 512;      // draws generated ahead (power of two)
+// the draw state of a wide run: the common layout (kRngWords words, last
+// word = 2) followed by the count of drawn-but-unused draws and the draws
+constexpr int kRngWordsWide = kRngWords + 1 + 2 * kWideRing;
+struct WideArgs {
+    int nwg;                 // workgroups of the launch (<= kMaxWide)
+    unsigned *sync;          // [1] round-barrier arrivals, zero at launch
+    int *err;                // [1] set when a round barrier gave up waiting
+    long long spin_ticks;    // give-up time of a barrier wait (s_memrealtime ticks)
+    double *bde;             // [kMaxWide] dE of each round slot
+    int4 *bmeta;             // [kMaxWide] (type index, first atom, moved atoms, instance)
+    double *bxyz;            // [kMaxWide * kMaxMoveAtoms * 3] trial coordinates
+    int4 *bip;               // [kMaxWide * kMaxMoveAtoms] trial fixed-point mirror
+};
+// Cooperative launch of nwg workgroups (all resident at once); returns the
+// launch status.
+hipError_t launch_mc_wide(const DevSys &s, const DevTerms &t, const DevMoves &mv, const MCRunArgs &a,
+                          const WideArgs &w, hipStream_t st);
+// workgroups of k_mc_wide that can be resident at once on the device
+int mc_wide_max_workgroups(int device);
 // adjoint.hip
 void launch_nonbonded_grad(const DevSys &s, double *g6, hipStream_t st);
 void launch_nonbonded_grad_batch(const DevSys *sys, double *const *g6, int nsys, int nmax, hipStream_t st);

@@ -324,38 +324,70 @@ def test_mc_replay_small_box():
     assert abs(ff.getE() - Oracle(ref).getE()[0]) <= 1e-6 * max(e0, abs(ro["etot"]))
 
 
-@pytest.mark.parametrize("case", ["water_every50_adapt", "water_auto_philox", "argon_philox_adapt"])
-def test_mc_read_ahead_same_chain(case, monkeypatch):
-    """Move sets without force-bias moves run the read-ahead kernel (the next
-    proposal drawn and loaded while the current move's pairs are evaluated);
-    MCX_SPECULATE=0 selects the one-proposal-at-a-time kernel.  Both give the
-    same chain bit for bit: the same dE, decisions, dmax and coordinates,
-    through list rebuilds, chunk boundaries (the draw state carried over)
-    and dmax adaptation steps (where no read-ahead is done)."""
-    import copy
-    if case.startswith("water"):
+KERNELS = {"one_at_a_time": {"MCX_SPECULATE": "0"},
+           "read_ahead": {"MCX_SPECULATE": "1", "MCX_WIDE": "0"},
+           "wide16": {"MCX_SPECULATE": "1", "MCX_WIDE": "16"},
+           "wide5": {"MCX_SPECULATE": "1", "MCX_WIDE": "5"},
+           "wide32": {"MCX_SPECULATE": "1", "MCX_WIDE": "32"}}
+
+
+def _chain_case(case):
+    if case.startswith("water_full"):
+        s, _ = systems.water_box(seed=5, nside=5, L=15.53, nmol=125, tag=41)
+        s.cutmode = systems.CUT_FULL
+    elif case.startswith("water_small_box"):
+        s, _ = systems.water_box(seed=21, nside=5, L=(90 / 0.0334) ** (1 / 3), nmol=90, temp=290.0, tag=300)
+    elif case.startswith("water"):
         s, _ = systems.water_box(seed=3, nside=8, L=24.85)
+    elif case.startswith("bonded"):
+        s, _ = systems.bonded_mix(seed=7)
     else:
         s, _ = systems.argon_fcc(seed=1)
     for k, m in enumerate(s.moves):
         m.each = 70 + 40 * k if case.endswith("adapt") else 0
+    return s
+
+
+@pytest.mark.parametrize("case", ["water_every50_adapt", "water_auto_philox", "argon_philox_adapt",
+                                  "bonded_every50_adapt", "water_full_cubic", "water_small_box_adapt"])
+def test_mc_chain_kernels_same_chain(case, monkeypatch):
+    """Move sets without force-bias moves run a speculative kernel: the
+    read-ahead kernel (one workgroup, the next proposal drawn and loaded
+    while the current move's pairs are evaluated; MCX_WIDE=0) or, by
+    default, the wide kernel (MCX_WIDE workgroups, each evaluating the
+    proposal some moves ahead on the assumption that the moves before it
+    are rejected, resolved in chain order).  MCX_SPECULATE=0 selects the
+    one-proposal-at-a-time kernel.  All give the same chain bit for bit: the
+    same dE, decisions, dmax and coordinates, through list rebuilds, chunk
+    boundaries (the draw state carried over), dmax adaptation steps (which
+    end a round), the displacement-triggered list (an accepted move past
+    half the skin ends the chunk), the all-pair instance and a box smaller
+    than twice the cutoff."""
+    import copy
+    s = _chain_case(case)
     mode = mcx.MC_REPLAY if "philox" not in case else mcx.MC_PHILOX
     every = mcx.LIST_AUTO if "auto" in case else 50
-    n = 4000
+    n = 3000
     out = {}
-    for spec in ("0", "1"):
-        monkeypatch.setenv("MCX_SPECULATE", spec)
+    for name, env in KERNELS.items():
+        for k in ("MCX_SPECULATE", "MCX_WIDE"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
         sc = copy.deepcopy(s)
         ff = mcx.FField(sc)
+        print("case", case, "kernel", name, flush=True)
         res = ff.h.mc_run(sc.moves, n, 97, rng_mode=mode, list_every=every)
-        out[spec] = (res, ff.get_coords())
-    (r0, c0), (r1, c1) = out["0"], out["1"]
+        out[name] = (res, ff.get_coords())
+    r0, c0 = out["one_at_a_time"]
     assert 0 < int(r0["acc"].sum()) < n
-    for key in ("acc", "de", "mvtyp", "mvinst", "disp", "trials", "accepted", "dmax"):
-        assert np.array_equal(r0[key], r1[key]), key
-    assert r0["etot"] == r1["etot"]
-    for a, b in zip(c0, c1):
-        assert np.array_equal(a, b)
+    for name in KERNELS:
+        r1, c1 = out[name]
+        for key in ("acc", "de", "mvtyp", "mvinst", "disp", "trials", "accepted", "dmax"):
+            assert np.array_equal(r0[key], r1[key]), (name, key)
+        assert r0["etot"] == r1["etot"], name
+        for a, b in zip(c0, c1):
+            assert np.array_equal(a, b), name
     if case.endswith("adapt"):
         assert not np.array_equal(r0["dmax"], np.array([m.dmax for m in s.moves]))
 
