@@ -29,7 +29,7 @@
 // row of capacity `cap` (full list, both directions; the j > i suffix from
 // half[i] on is the reference's half row).  Each wavefront compacts every
 // fourth row of the group.
-#include "mcx_internal.h"
+#include "listcore.h"
 
 #include <cmath>
 
@@ -43,25 +43,9 @@ constexpr int kUnroll = 4;
 constexpr int kRowWaves = 4;       // wavefronts per row-group workgroup
 constexpr double kBand = 1e-5;     // relative band around (ctoff+dcut)^2 decided in FP64
 
-__device__ __forceinline__ double ref_rint(double v)
-{
-    return (double)(v >= 0. ? (int)(v + 0.5) : (int)(v - 0.5));
-}
-
-// wrapping int32 difference b - a: the minimum image in units of L/2^32
-__device__ __forceinline__ int wdiff(int b, int a) { return (int)((unsigned)b - (unsigned)a); }
-
 __global__ void k_zero_int(int *p, int n)
 {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) p[k] = 0;
-}
-
-__device__ __forceinline__ int cell_index(double x, double rL, int nc)
-{
-    double u = x * rL;
-    u = u - floor(u);
-    int c = (int)(u * (double)nc);
-    return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
 }
 
 __device__ __forceinline__ int cell_of_atom(const DevSys &s, const CellGrid &g, int a)
@@ -202,22 +186,6 @@ __global__ __launch_bounds__(256) void k_cell_sort(DevSys s, CellGrid g)
     }
 }
 
-// Tools::distance2 + the r2 <= cutnb2 test, operation by operation
-template <bool PBC>
-__device__ __forceinline__ bool within(const DevSys &s, double4 pj, double xi, double yi, double zi)
-{
-    double dx = pj.x - xi;
-    double dy = pj.y - yi;
-    double dz = pj.z - zi;
-    if (PBC) {
-        dx -= s.L[0] * ref_rint(s.rL[0] * dx);
-        dy -= s.L[1] * ref_rint(s.rL[1] * dy);
-        dz -= s.L[2] * ref_rint(s.rL[2] * dz);
-    }
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    return r2 <= s.cutnb2;
-}
-
 // Candidate runs of the cells around cell ci: thread t < 32 takes (y, z)
 // column t of neighbour cells; the x cells of a column are contiguous in
 // cell order, except where the range wraps (then two runs).  Returns the
@@ -253,19 +221,6 @@ __device__ __forceinline__ int cell_runs(const CellGrid &g, int ci, int t, int *
         run_start[2 * col + 1] = s1; run_len[2 * col + 1] = l1;
     }
     return ncol;
-}
-
-// Inclusive prefix sum over the 64 lanes of a wavefront: row_shr 1, 2, 4, 8
-// within rows of 16 lanes, then row_bcast 15 and 31 carry the row totals.
-__device__ __forceinline__ int wave_incl_scan(int v)
-{
-    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
-    return v;
 }
 
 // Ordered compaction of row i's bitmap B (LDS) into its ELL row; sets cnt,

@@ -162,11 +162,14 @@ __device__ __forceinline__ bool set_holds(const MoveLDS &m, const int4 id)
 
 // Nonbonded part (threads 64..): raw Coulomb and LJ sums, and with `fb` the
 // net force on the set at its new position.
+// pt: the pair thread (0 .. kMCT-65) whose entries this thread evaluates;
+// the one-workgroup kernels pass threadIdx.x - 64, the wide kernel the
+// thread of the wavefront it stands in for when a proposal's pairs are
+// split over several workgroups.
 __device__ __forceinline__ void delta_pairs(const DevSys &s, const MoveLDS &m, bool fb, double &dec, double &dev,
-                                            double (&fnew)[3])
+                                            double (&fnew)[3], int pt = threadIdx.x - 64)
 {
     constexpr int NP = kMCT - 64;
-    const int pt = threadIdx.x - 64;
     const int total = m.total, nm = m.nm;
     const bool all = s.cutmode == 0 || m.scan;
     // Row starts of a set of up to four atoms, read once: the moved atom of
@@ -340,6 +343,15 @@ __device__ __forceinline__ void load_moved(const DevSys &s, const DevMoves &mv, 
     __syncthreads();
 }
 
+// Records atom i as moved for the incremental list update: in the moves
+// since the last update and in those since the last full build.
+__device__ __forceinline__ void record_move(const MCRunArgs &a, int i)
+{
+    const ListPatch &p = a.patch;
+    if (atomicExch(&p.moved_flag[i], 1) == 0) p.moved_list[atomicAdd(p.moved_cnt, 1)] = i;
+    if (atomicExch(&p.stale_flag[i], 1) == 0) p.stale_list[atomicAdd(p.stale_cnt, 1)] = i;
+}
+
 // Is fixed-point position p farther than half the list skin from its
 // position r at the last list build?
 __device__ __forceinline__ bool beyond_half_skin(const DevSys &s, int4 p, int4 r, double half_skin2)
@@ -494,7 +506,7 @@ template <int PB, int CUT>
 __global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s0, DevTerms t, DevMoves mv, MCRunArgs arg)
 {
     const DevSys s = specialise<PB, CUT>(s0);
-    if (arg.bail && (s.overflow[0] != 0 || arg.bail[0] != INT_MAX)) {
+    if (arg.bail && ((s.overflow && s.overflow[0] != 0) || arg.bail[0] != INT_MAX)) {
         // the list this chunk would use is incomplete: leave every state
         // untouched; the host grows the list and runs again from here
         if (threadIdx.x == 0) atomicMin(arg.bail, arg.step0);
@@ -613,6 +625,7 @@ __global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s0, DevTerms t, DevMov
                 const int4 np = mcx_fixed4(s, m.tx[lane], m.ty[lane], m.tz[lane], __int_as_float(m.ip[lane].w));
                 s.ipos[i] = np;
                 far = arg.ipos_ref && beyond_half_skin(s, np, arg.ipos_ref[i], arg.half_skin2);
+                if (arg.record_moves) record_move(arg, i);
             }
             if (__ballot(far) != 0ull && lane == 0) m.stop = 1;
         }
@@ -687,7 +700,7 @@ template <int PB, int CUT>
 __global__ __launch_bounds__(kMCT) void k_mc_spec(DevSys s0, DevTerms t, DevMoves mv, MCRunArgs arg)
 {
     const DevSys s = specialise<PB, CUT>(s0);
-    if (arg.bail && (s.overflow[0] != 0 || arg.bail[0] != INT_MAX)) {
+    if (arg.bail && ((s.overflow && s.overflow[0] != 0) || arg.bail[0] != INT_MAX)) {
         // the list this chunk would use is incomplete: leave every state
         // untouched; the host grows the list and runs again from here
         if (threadIdx.x == 0) atomicMin(arg.bail, arg.step0);
@@ -778,6 +791,7 @@ __global__ __launch_bounds__(kMCT) void k_mc_spec(DevSys s0, DevTerms t, DevMove
                     const int4 np = mcx_fixed4(s, m.tx[lane], m.ty[lane], m.tz[lane], __int_as_float(m.ip[lane].w));
                     s.ipos[i] = np;
                     far = arg.ipos_ref && beyond_half_skin(s, np, arg.ipos_ref[i], arg.half_skin2);
+                    if (arg.record_moves) record_move(arg, i);
                 }
                 if (__ballot(far) != 0ull) stop = true;
                 // An acceptance that took its acceptance draw (dE above
@@ -882,6 +896,8 @@ __device__ __forceinline__ void skip_proposal(const DevMoves &mv, D &dr, int lan
     if (type == MCX_MV_ROT_DEV) (void)dr.next(lane);
 }
 
+constexpr int kWideRec = 256;         // accepted moves a launch records in LDS (more: recorded at once)
+
 struct WideLDS {
     MoveLDS m;                         // this workgroup's proposal
     double ring[kWideRing];
@@ -923,6 +939,23 @@ __device__ __forceinline__ bool round_barrier(const WideArgs &w, unsigned target
     return s_fail == 0;
 }
 
+// End of a k_mc_wide workgroup: the last one to finish clears the launch's
+// counters for the next launch (every workgroup has passed its last
+// rendezvous or barrier by then).
+__device__ __forceinline__ void wide_exit(const WideArgs &w)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (__hip_atomic_fetch_add(w.exit_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == (unsigned)w.nwg - 1) {
+            __hip_atomic_store(w.sync, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(w.arrive, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(w.go, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(w.exit_ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
 // First step >= st0 (1-based) after which dmax is adapted, or INT_MAX.
 __device__ __forceinline__ long long next_adapt(const DevMoves &mv, long long st0)
 {
@@ -940,7 +973,7 @@ template <int PB, int CUT>
 __global__ __launch_bounds__(kMCT) void k_mc_wide(DevSys s0, DevTerms t, DevMoves mv, MCRunArgs arg, WideArgs w)
 {
     const DevSys s = specialise<PB, CUT>(s0);
-    if (arg.bail && (s.overflow[0] != 0 || arg.bail[0] != INT_MAX)) {
+    if (arg.bail && ((s.overflow && s.overflow[0] != 0) || arg.bail[0] != INT_MAX)) {
         // the list this chunk would use is incomplete: leave every state
         // untouched; the host grows the list and runs again from here
         if (threadIdx.x == 0 && blockIdx.x == 0) atomicMin(arg.bail, arg.step0);
@@ -952,15 +985,55 @@ __global__ __launch_bounds__(kMCT) void k_mc_wide(DevSys s0, DevTerms t, DevMove
     __shared__ int acc_tmp[kMaxMoveTypes];
     __shared__ long long n_trial[kMaxMoveTypes], n_acc[kMaxMoveTypes];
     __shared__ double etot;
-    __shared__ int s_stop, s_done;
+    __shared__ int s_stop, s_done, s_go;
+    __shared__ int2 rec[kWideRec];      // accepted moves (first atom, atoms) to record at the end
+    __shared__ int rec_pref[kWideRec + 1];
+    __shared__ int s_nrec;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int me = blockIdx.x;
+    const int prop = me / w.split, part = me % w.split;   // this workgroup's proposal and share of its pairs
     const bool w0 = __builtin_amdgcn_readfirstlane(tid) < 64;   // wave-uniform
     MoveLDS &m = g.m;
+    // Rendezvous: the rounds need every workgroup resident at once.  The
+    // last to arrive opens it; one that waits longer than rdv_ticks closes
+    // it instead; the first of the two decisions holds for all.  A closed
+    // rendezvous changes nothing: the host runs the chunk again.
+    if (tid == 0) {
+        const unsigned t = __hip_atomic_fetch_add(w.arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int go = 0;
+        if (t == (unsigned)w.nwg - 1) {
+            int expect = 0;
+            go = __hip_atomic_compare_exchange_strong(w.go, &expect, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT) ? 1 : expect;
+        } else {
+            const long long t0 = (long long)wall_clock64();
+            while ((go = __hip_atomic_load(w.go, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 0) {
+                if ((long long)wall_clock64() - t0 > w.rdv_ticks) {
+                    int expect = 0;
+                    go = __hip_atomic_compare_exchange_strong(w.go, &expect, 2, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_AGENT) ? 2 : expect;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        s_go = go;
+    }
+    __syncthreads();
+    if (s_go != 1) {
+        if (tid == 0 && me == 0) {
+            __hip_atomic_store(w.err, 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (arg.bail) atomicMin(arg.bail, arg.step0);
+            if (arg.stop_at) { arg.stop_at[0] = 0; arg.stop_at[1] = 0; }
+        }
+        wide_exit(w);
+        return;
+    }
     if (arg.rng_mode == 1)
         for (int k = tid; k < kMTN; k += kMCT) mt[k] = arg.mt_state[k];
-    if (tid == 0) { s_stop = 0; s_done = 0; g.fail = 0; }
+    if (tid == 0) { s_stop = 0; s_done = 0; g.fail = 0; s_nrec = 0; }
     __syncthreads();
     WaveDraws gen;
     long long pos = 0, filled = 0;
@@ -1044,7 +1117,7 @@ __global__ __launch_bounds__(kMCT) void k_mc_wide(DevSys s0, DevTerms t, DevMove
             const int left = arg.nsteps - done;
             if (eq_hi < pos) eq_hi = pos;
             if (left > 0 && !s_stop) {
-                long long wmax = min(left, w.nwg);
+                long long wmax = min(left, w.nwg / w.split);
                 const long long na = next_adapt(mv, (long long)arg.step0 + done + 1);
                 if (na != LLONG_MAX) wmax = min(wmax, na - ((long long)arg.step0 + done));
                 long long p = pos;
@@ -1088,35 +1161,46 @@ __global__ __launch_bounds__(kMCT) void k_mc_wide(DevSys s0, DevTerms t, DevMove
             if (lane == 0) g.nprop = np;
             wave_sync();
             // this workgroup's proposal: draw and load it, form the trial
-            if (me < np) {
-                RingDraws d{g.ring, g.start[me], filled, true};
+            if (prop < np) {
+                RingDraws d{g.ring, g.start[prop], filled, true};
                 pick_load(s, mv, d, dmax, m, lane, arg.ipos_ref, arg.half_skin2);
             }
         }
         __syncthreads();
         const int np = g.nprop;
         if (np == 0) break;
-        // ---- dE of this workgroup's proposal; meanwhile wavefront 0 fills
-        // the draw ring and parses ahead for the next round
-        if (me < np) {
+        // ---- dE of this workgroup's proposal (its share of the pairs when
+        // a proposal is split over several workgroups): each wavefront's
+        // sums go to the board, where the resolution adds them in the
+        // order of delta_finish; meanwhile wavefront 0 fills the draw ring
+        // and parses ahead for the next round
+        if (prop < np) {
             double dec = 0.0, dev = 0.0, deb = 0.0;
             double fnew[3] = {0.0, 0.0, 0.0};
+            const int v0 = 1 + part * (kMCW - 1) / w.split, v1 = 1 + (part + 1) * (kMCW - 1) / w.split;
+            const int wr = tid >> 6;                  // this wavefront; wavefront 0 keeps the bonded terms
+            const int vw = v0 + wr - 1;               // the wavefront of the one-workgroup kernel it stands in for
             if (w0) {
-                delta_terms(s, mv, m, dec, dev, deb);
+                if (part == 0) delta_terms(s, mv, m, dec, dev, deb);
                 parse_ahead();
-            } else {
-                delta_pairs(s, m, false, dec, dev, fnew);
+            } else if (vw < v1) {
+                delta_pairs(s, m, false, dec, dev, fnew, (vw - 1) * 64 + lane);
             }
-            delta_finish(m, false, dec, dev, deb, fnew);
-            if (w0) {
+            if (w0 ? part == 0 : vw < v1) {
+                dec = wsum_dpp(dec);
+                dev = wsum_dpp(dev);
+                deb = wsum_dpp(deb);
                 if (lane == 0) {
-                    w.bde[me] = m.de;
-                    w.bmeta[me] = make_int4(m.acc, m.first, m.nm, m.inst);
+                    double *r = w.bred + ((size_t)prop * kMCW + (w0 ? 0 : vw)) * 3;
+                    r[0] = dec; r[1] = dev; r[2] = deb;
                 }
+            }
+            if (w0 && part == 0) {
+                if (lane == 0) w.bmeta[prop] = make_int4(m.acc, m.first, m.nm, m.inst);
                 for (int a = lane; a < m.nm; a += 64) {
-                    double *b = w.bxyz + ((size_t)me * kMaxMoveAtoms + a) * 3;
+                    double *b = w.bxyz + ((size_t)prop * kMaxMoveAtoms + a) * 3;
                     b[0] = m.tx[a]; b[1] = m.ty[a]; b[2] = m.tz[a];
-                    w.bip[(size_t)me * kMaxMoveAtoms + a] =
+                    w.bip[(size_t)prop * kMaxMoveAtoms + a] =
                         mcx_fixed4(s, m.tx[a], m.ty[a], m.tz[a], __int_as_float(m.ip[a].w));
                 }
             }
@@ -1130,7 +1214,16 @@ __global__ __launch_bounds__(kMCT) void k_mc_wide(DevSys s0, DevTerms t, DevMove
         // also reads its first four trial positions, so that the commit of
         // a move of up to four atoms needs no further round trip.
         if (w0) {
-            const double de_l = lane < np ? w.bde[lane] : 0.0;
+            // lane q < np: proposal q's dE, its wavefront sums added in the
+            // order of delta_finish
+            double de_l = 0.0;
+            if (lane < np) {
+                const double *r = w.bred + (size_t)lane * kMCW * 3;
+                double c = 0.0, v = 0.0, bb = 0.0;
+#pragma unroll
+                for (int k = 0; k < kMCW; k++) { c += r[3 * k]; v += r[3 * k + 1]; bb += r[3 * k + 2]; }
+                de_l = MCX_CHGCHARMM * MCX_KCALTOIU * c + 4.0 * v + bb;
+            }
             const int4 meta_l = lane < np ? w.bmeta[lane] : make_int4(0, 0, 0, 0);
             const int pq = lane >> 2, pa = lane & 3;
             double cx = 0.0, cy = 0.0, cz = 0.0;
@@ -1146,22 +1239,35 @@ __global__ __launch_bounds__(kMCT) void k_mc_wide(DevSys s0, DevTerms t, DevMove
             const uint64_t accm = __ballot(acc_l);
             const int acc_at = accm ? __builtin_ctzll(accm) : -1;
             const int R = acc_at >= 0 ? acc_at + 1 : np;
-            if (lane == 0) {
-                for (int q = 0; q < R; q++) {
-                    const double de = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(de_l), q),
-                                                       __builtin_amdgcn_readlane(__double2loint(de_l), q));
-                    const int tp = __builtin_amdgcn_readlane(meta_l.x, q);
-                    const int acc = q == acc_at;
-                    n_trial[tp]++;
-                    if (acc) { etot += de; n_acc[tp]++; }
-                    adapt_dmax(mv, (long long)arg.step0 + done + q + 1, tp, acc, dmax, acc_tmp);
-                }
+            // bookkeeping of the R resolved moves: trials per type, the
+            // accepted move, then the dmax adaptation of MC::adjust_dmax
+            // (src/MC.cpp:165-173), which a round's last move alone can meet
+            // (rounds end at adaptation steps)
+            int ntr = 0;
+            for (int k = 0; k < mv.ntypes; k++) {
+                const int c = __popcll(__ballot(lane < R && meta_l.x == k));
+                if (lane == k) ntr = c;
             }
-            if (me < R && lane == 0) {
+            if (lane < mv.ntypes) n_trial[lane] += ntr;
+            wave_sync();
+            if (lane == 0) {
+                if (acc_at >= 0) {
+                    const int tp = __builtin_amdgcn_readlane(meta_l.x, acc_at);
+                    etot += __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(de_l), acc_at),
+                                             __builtin_amdgcn_readlane(__double2loint(de_l), acc_at));
+                    n_acc[tp]++;
+                    acc_tmp[tp]++;
+                }
+                adapt_dmax(mv, (long long)arg.step0 + done + R, __builtin_amdgcn_readlane(meta_l.x, R - 1), 0, dmax,
+                           acc_tmp);
+            }
+            const double de_own = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(de_l), prop < np ? prop : 0),
+                                                   __builtin_amdgcn_readlane(__double2loint(de_l), prop < np ? prop : 0));
+            if (prop < R && part == 0 && lane == 0) {
                 // this workgroup's own move: its per-step records
-                const int o = arg.step0 + done + me;
-                if (arg.acc_out) arg.acc_out[o] = (unsigned char)(me == acc_at);
-                if (arg.de_out) arg.de_out[o] = m.de;
+                const int o = arg.step0 + done + prop;
+                if (arg.acc_out) arg.acc_out[o] = (unsigned char)(prop == acc_at);
+                if (arg.de_out) arg.de_out[o] = de_own;
                 if (arg.mvtyp_out) arg.mvtyp_out[o] = m.acc;
                 if (arg.mvinst_out) arg.mvinst_out[o] = m.inst;
                 if (arg.lnq_out) arg.lnq_out[o] = 0.0;
@@ -1197,6 +1303,17 @@ __global__ __launch_bounds__(kMCT) void k_mc_wide(DevSys s0, DevTerms t, DevMove
                     s.ipos[i] = np4;
                     far = far || (arg.ipos_ref && beyond_half_skin(s, np4, arg.ipos_ref[i], arg.half_skin2));
                 }
+                // the moved atoms, recorded for the list update at the end of
+                // the launch (off this round's critical path)
+                if (arg.record_moves && me == 0) {
+                    if (s_nrec < kWideRec) {
+                        if (lane == 0) rec[s_nrec] = make_int2(first, nm);
+                        wave_sync();
+                        if (lane == 0) s_nrec++;
+                    } else {
+                        for (int a = lane; a < nm; a += 64) record_move(arg, first + a);
+                    }
+                }
             }
             done += R;
             if (__ballot(far) != 0ull && lane == 0) s_stop = 1;
@@ -1206,13 +1323,34 @@ __global__ __launch_bounds__(kMCT) void k_mc_wide(DevSys s0, DevTerms t, DevMove
         __syncthreads();
         done = s_done;
     }
-    if (!ok) return;
+    if (!ok) {
+        wide_exit(w);
+        return;
+    }
     if (arg.stop_at && tid == 0 && me == 0) {
         arg.stop_at[0] = done;
         arg.stop_at[1] = s_stop;
     }
+    wide_exit(w);
     if (me != 0) return;
-    __syncthreads();
+    if (arg.record_moves) {
+        // the launch's moved atoms, one thread each
+        if (tid == 0) {
+            int acc = 0;
+            for (int r = 0; r < s_nrec; r++) { rec_pref[r] = acc; acc += rec[r].y; }
+            rec_pref[s_nrec] = acc;
+        }
+        __syncthreads();
+        const int nr = s_nrec, total = rec_pref[nr];
+        for (int t = tid; t < total; t += kMCT) {
+            int lo = 0, hi = nr - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (rec_pref[mid] <= t) lo = mid; else hi = mid - 1;
+            }
+            record_move(arg, rec[lo].x + (t - rec_pref[lo]));
+        }
+    }
     if (w0) {
         if (lane < mv.ntypes) {
             arg.dmax[lane] = dmax[lane]; arg.acc_tmp[lane] = acc_tmp[lane];
@@ -1285,15 +1423,13 @@ hipError_t launch_mc_wide(const DevSys &s, const DevTerms &t, const DevMoves &mv
     DevMoves mv_ = mv;
     MCRunArgs a_ = a;
     WideArgs w_ = w;
-    void *args[] = {&s_, &t_, &mv_, &a_, &w_};
-    const void *f;
     if (s.pbtype == MCX_PBC_CUBIC && s.cutmode == MCX_CUT_SWITCH)
-        f = reinterpret_cast<const void *>(&k_mc_wide<MCX_PBC_CUBIC, MCX_CUT_SWITCH>);
+        hipLaunchKernelGGL((k_mc_wide<MCX_PBC_CUBIC, MCX_CUT_SWITCH>), dim3(w.nwg), dim3(kMCT), 0, st, s_, t_, mv_, a_, w_);
     else if (s.pbtype == MCX_PBC_CUBIC && s.cutmode == MCX_CUT_FULL)
-        f = reinterpret_cast<const void *>(&k_mc_wide<MCX_PBC_CUBIC, MCX_CUT_FULL>);
+        hipLaunchKernelGGL((k_mc_wide<MCX_PBC_CUBIC, MCX_CUT_FULL>), dim3(w.nwg), dim3(kMCT), 0, st, s_, t_, mv_, a_, w_);
     else
-        f = reinterpret_cast<const void *>(&k_mc_wide<-1, -1>);
-    return hipLaunchCooperativeKernel(f, dim3(w.nwg), dim3(kMCT), args, 0, st);
+        hipLaunchKernelGGL((k_mc_wide<-1, -1>), dim3(w.nwg), dim3(kMCT), 0, st, s_, t_, mv_, a_, w_);
+    return hipGetLastError();
 }
 
 int mc_wide_max_workgroups(int device)

@@ -266,6 +266,12 @@ long long build_list(mcx_ctx *h)
         HIPCHK(hipStreamSynchronize(h->stream));
         if (ov == 0) {
             h->list_valid = true;
+            if (h->patch_alloc) {
+                // a fresh binning: nothing has moved since this build
+                launch_list_patch_reset(h->sys(), h->patch(), h->stream);
+                HIPCHK(hipGetLastError());
+                h->patch_valid = true;
+            }
             // count the half pairs
             std::vector<int> c(h->n), hf(h->n);
             HIPCHK(hipMemcpy(c.data(), h->cnt.p, sizeof(int) * h->n, hipMemcpyDeviceToHost));
@@ -291,7 +297,29 @@ void rebuild_list_fast(mcx_ctx *h)
     if (ov) {
         ensure_list_capacity(h, (int)(ov * 1.25) + 32);
         build_list(h);
+    } else if (h->patch_alloc) {
+        launch_list_patch_reset(h->sys(), h->patch(), h->stream);
+        HIPCHK(hipGetLastError());
+        h->patch_valid = true;
     }
+}
+
+// Buffers of the incremental list update, allocated once per handle (the
+// rebuilt-row buffer grows with the list capacity); new state starts empty.
+void ensure_patch(mcx_ctx *h)
+{
+    const size_t n = (size_t)h->n;
+    const size_t rows = std::min(n, (size_t)kPatchRows);
+    h->p_newrow.alloc(rows * (size_t)std::max(h->cap, 1));
+    if (h->patch_alloc) return;
+    h->p_moved_flag.alloc(n); h->p_moved_list.alloc(n); h->p_moved_cnt.alloc(1);
+    h->p_stale_flag.alloc(n); h->p_stale_list.alloc(n); h->p_stale_cnt.alloc(1);
+    h->p_edit_cnt.alloc(n); h->p_edit_val.alloc(n * kPatchEdits);
+    h->p_touched.alloc(n); h->p_touched_cnt.alloc(1);
+    h->p_newcnt.alloc(rows); h->p_newhalf.alloc(rows);
+    h->p_ticket.alloc(1);
+    h->patch_alloc = true;
+    h->patch_valid = false;
 }
 
 void setup_tiles(mcx_ctx *h)
@@ -902,6 +930,19 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
         if (list_every < 0 && list_every != MCX_LIST_AUTO) throw mcx_error(MCX_ERR_ARG, "list_every < 0");
         if (autolist && h->pbtype == MCX_PBC_NONE)
             throw mcx_error(MCX_ERR_UNSUPPORTED, "displacement-triggered lists need a periodic box");
+        // scheduled rebuilds as incremental updates (the same list as a full
+        // rebuild) unless MCX_LIST_PATCH=0; they need every move since the
+        // last full build recorded, so a run that does not record leaves the
+        // next one a full build
+        const char *lp = std::getenv("MCX_LIST_PATCH");
+        const bool use_patch = list_every > 0 && h->cutmode != MCX_CUT_FULL && !(lp && lp[0] == '0') &&
+                               h->list_valid && list_patch_supported(h->sys());
+        if (use_patch) {
+            ensure_patch(h);
+            if (!h->patch_valid) build_list(h);
+        } else {
+            h->patch_valid = false;
+        }
         if (h->cutmode != MCX_CUT_FULL && !h->list_valid) build_list(h);
         DevMoves mv = build_moves(h, types, ntypes, h->mvb, kMaxMoveAtoms);
         // device-side run state
@@ -953,30 +994,44 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
         // chain) unless a force-bias type is present or MCX_SPECULATE=0
         const char *sp = std::getenv("MCX_SPECULATE");
         a.spec = !(sp && sp[0] == '0');
+        if (use_patch) {
+            a.patch = h->patch();
+            a.record_moves = 1;
+        }
         for (int k = 0; k < ntypes; k++)
             if (types[k].type == MCX_MV_FBIAS) a.spec = 0;
         // speculative rounds over several workgroups (k_mc_wide) for the
         // same move sets; MCX_WIDE=<n> sets the workgroup count, 0 or 1
         // keeps the single-workgroup read-ahead kernel
-        int nwg = 0;
+        // MCX_WIDE=<n> proposals per round, MCX_WIDE_SPLIT=<k> workgroups per
+        // proposal (its pair wavefronts shared out between them)
+        int nwg = 0, split = 1;
         if (a.spec) {
             const char *wv = std::getenv("MCX_WIDE");
+            const char *sv = std::getenv("MCX_WIDE_SPLIT");
             const int want = wv ? std::atoi(wv) : kDefaultWide;
-            if (want > 1) nwg = std::min(std::min(want, kMaxWide), mc_wide_max_workgroups(h->device));
-            if (nwg < 2) nwg = 0;
+            split = std::max(1, std::min(sv ? std::atoi(sv) : kDefaultSplit, 15));
+            const int cap = std::min(kMaxWide, mc_wide_max_workgroups(h->device)) / split;
+            const int W = std::min(want, cap);
+            if (want > 1 && W >= 2) nwg = W * split;
         }
-        DBuf<unsigned> sync_d;
-        DBuf<int> werr_d;
-        DBuf<double> bde_d, bxyz_d;
+        DBuf<unsigned> sync_d, exit_d, arrive_d;
+        DBuf<int> werr_d, go_d;
+        DBuf<double> bred_d, bxyz_d;
         DBuf<int4> bmeta_d, bip_d;
-        cleanup.f.push_back([&] { sync_d.release(); werr_d.release(); bde_d.release(); bxyz_d.release();
+        cleanup.f.push_back([&] { sync_d.release(); exit_d.release(); arrive_d.release(); go_d.release();
+                                  werr_d.release(); bred_d.release(); bxyz_d.release();
                                   bmeta_d.release(); bip_d.release(); });
         WideArgs wa{};
         if (nwg) {
-            sync_d.alloc(1);
+            const unsigned uzero = 0u;
+            sync_d.upload(&uzero, 1, h->stream);
+            exit_d.upload(&uzero, 1, h->stream);
+            arrive_d.upload(&uzero, 1, h->stream);
             const int zero = 0;
             werr_d.upload(&zero, 1, h->stream);
-            bde_d.alloc(kMaxWide);
+            go_d.upload(&zero, 1, h->stream);
+            bred_d.alloc((size_t)kMaxWide * 16 * 3);
             bmeta_d.alloc(kMaxWide);
             bxyz_d.alloc((size_t)kMaxWide * kMaxMoveAtoms * 3);
             bip_d.alloc((size_t)kMaxWide * kMaxMoveAtoms);
@@ -984,25 +1039,41 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
             HIPCHK(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, h->device));
             wa.nwg = nwg;
             wa.sync = sync_d.p;
+            wa.exit_ctr = exit_d.p;
             wa.err = werr_d.p;
             wa.spin_ticks = (long long)std::max(rate_khz, 1000) * 500;    // half a second
-            wa.bde = bde_d.p; wa.bmeta = bmeta_d.p; wa.bxyz = bxyz_d.p; wa.bip = bip_d.p;
+            wa.rdv_ticks = (long long)std::max(rate_khz, 1000) * 20;      // 20 ms
+            wa.arrive = arrive_d.p;
+            wa.go = go_d.p;
+            wa.split = split;
+            wa.bred = bred_d.p; wa.bmeta = bmeta_d.p; wa.bxyz = bxyz_d.p; wa.bip = bip_d.p;
         }
         auto launch_chunk = [&]() {
             if (nwg) {
-                HIPCHK(hipMemsetAsync(sync_d.p, 0, sizeof(unsigned), h->stream));
                 HIPCHK(launch_mc_wide(h->sys(), h->dterms(), mv, a, wa, h->stream));
             } else {
                 launch_mc_chunk(h->sys(), h->dterms(), mv, a, h->stream);
             }
         };
-        auto check_wide = [&]() {
-            if (!nwg) return;
+        // true: a launch found its workgroups not resident together and
+        // changed nothing (it set the bail-out step); it is run again
+        int wide_retries = 0;
+        auto check_wide = [&]() -> bool {
+            if (!nwg) return false;
             int e = 0;
             HIPCHK(hipMemcpyAsync(&e, werr_d.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
             HIPCHK(hipStreamSynchronize(h->stream));
+            if (e == 3) {
+                if (++wide_retries > 64)
+                    throw mcx_error(MCX_ERR_HIP, "the speculative kernel's workgroups were never resident together "
+                                                 "(a busy device; MCX_WIDE=0 avoids it)");
+                const int zero = 0;
+                werr_d.upload(&zero, 1, h->stream);
+                return true;
+            }
             if (e == 2) throw mcx_error(MCX_ERR_HIP, "a proposal needed more draws than the speculative kernel holds");
             if (e) throw mcx_error(MCX_ERR_HIP, "a speculative round barrier gave up waiting (MCX_WIDE=0 avoids it)");
+            return false;
         };
         a.etot = etot.p; a.mt_state = mts.p; a.dmax = dmax.p; a.trials = trials.p; a.accepted = accepted.p;
         a.acc_tmp = acc_tmp.p;
@@ -1045,7 +1116,7 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
             a.step0 = done;
             launch_chunk();
             HIPCHK(hipGetLastError());
-            check_wide();
+            if (check_wide()) continue;
             int st[2] = {0, 0};
             HIPCHK(hipMemcpyAsync(st, stop_d.p, sizeof(st), hipMemcpyDeviceToHost, h->stream));
             HIPCHK(hipStreamSynchronize(h->stream));
@@ -1067,7 +1138,9 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
         // that bailed out.
         constexpr int kCheckEvery = 16;
         const int kNone = INT_MAX;
-        if (!autolist && rebuilds) {
+        // the bail-out record also lets a wide launch that found its
+        // workgroups not resident together hand its chunk back
+        if (!autolist && (rebuilds || nwg)) {
             bail_d.upload(&kNone, 1, h->stream);
             a.bail = bail_d.p;
         }
@@ -1080,8 +1153,10 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
             HIPCHK(hipStreamSynchronize(h->stream));
             if (b != kNone) {
                 done = b;
-                build_list(h);               // grows the capacity until the rows fit
-                nlist++;
+                if (rebuilds) {
+                    build_list(h);           // grows the capacity until the rows fit
+                    nlist++;
+                }
                 bail_d.upload(&kNone, 1, h->stream);
             }
             since_check = 0;
@@ -1096,7 +1171,12 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
             HIPCHK(hipGetLastError());
             done += chunk;
             if (rebuilds && done % list_every == 0) {
-                launch_list_build(h->sys(), h->grid, h->stream);
+                if (use_patch) {
+                    h->p_newrow.alloc((size_t)std::min(h->n, kPatchRows) * h->cap);   // grows with the capacity
+                    launch_list_patch(h->sys(), h->grid, h->patch(), h->stream);
+                } else {
+                    launch_list_build(h->sys(), h->grid, h->stream);
+                }
                 HIPCHK(hipGetLastError());
                 nlist++;
             }
@@ -1274,7 +1354,7 @@ int mcx_time_kernel(mcx_ctx *h, int which, int reps, double *ms)
             switch (which) {
             case 0: launch_full_energy(s, h->tiles.p, h->ntiles, kFullTile, h->partial.p, h->stream); break;
             case 1: launch_switch_energy(s, h->partial.p, nbs, h->stream); break;
-            case 2: launch_list_build(s, h->grid, h->stream); break;
+            case 2: launch_list_build(s, h->grid, h->stream); h->patch_valid = false; break;
             case 3: launch_bonded(s, t, h->partial.p, bonded_blocks(t), h->stream); break;
             case 4: launch_nonbonded_grad(s, g6.p, h->stream); break;
             default: throw mcx_error(MCX_ERR_ARG, "unknown kernel id");

@@ -95,6 +95,24 @@ struct mcx_ctx {
     uint32_t rng_stream = 0;   // Philox stream of mcx_mc_run
     DBuf<double> newxyz, scalar;
     long long last_npairs = 0;
+    // incremental list updates (list_patch.hip); valid while every move since
+    // the last full build was recorded
+    DBuf<int> p_moved_flag, p_moved_list, p_moved_cnt, p_stale_flag, p_stale_list, p_stale_cnt;
+    DBuf<int> p_edit_cnt, p_edit_val, p_touched, p_touched_cnt, p_newrow, p_newcnt, p_newhalf;
+    DBuf<unsigned> p_ticket;
+    bool patch_alloc = false, patch_valid = false;
+
+    ListPatch patch() const
+    {
+        ListPatch p{};
+        p.moved_flag = p_moved_flag.p; p.moved_list = p_moved_list.p; p.moved_cnt = p_moved_cnt.p;
+        p.stale_flag = p_stale_flag.p; p.stale_list = p_stale_list.p; p.stale_cnt = p_stale_cnt.p;
+        p.edit_cnt = p_edit_cnt.p; p.edit_val = p_edit_val.p;
+        p.touched = p_touched.p; p.touched_cnt = p_touched_cnt.p;
+        p.newrow = p_newrow.p; p.newcnt = p_newcnt.p; p.newhalf = p_newhalf.p;
+        p.ticket = p_ticket.p;
+        return p;
+    }
 
     DevSys sys() const
     {
@@ -165,6 +183,10 @@ struct mcx_ctx {
                            &cell_start, &cell_count, &cell_atoms, &cursor, &group_start, &cell_tmp, &adj_off, &adj_kk};
         for (auto *b : ib) b->release();
         tiles.release();
+        DBuf<int> *pb[] = {&p_moved_flag, &p_moved_list, &p_moved_cnt, &p_stale_flag, &p_stale_list, &p_stale_cnt,
+                           &p_edit_cnt, &p_edit_val, &p_touched, &p_touched_cnt, &p_newrow, &p_newcnt, &p_newhalf};
+        for (auto *b : pb) b->release();
+        p_ticket.release();
         for (int k = 0; k < kNumKinds; k++) { t_idx[k].release(); t_type[k].release(); t_par[k].release(); }
         if (stream) (void)hipStreamDestroy(stream);
     }

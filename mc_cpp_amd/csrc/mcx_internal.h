@@ -171,6 +171,32 @@ struct CellGrid {
 };
 void launch_list_build(const DevSys &s, const CellGrid &g, hipStream_t st);
 size_t list_build_lds_bytes(int n);
+// list_patch.hip: the list brought up to date after Metropolis moves by
+// recomputing only what they can change.  Atoms moved since the last
+// update get their rows rebuilt; every other row only gains or loses moved
+// atoms, applied as sorted edits.  The result equals a full rebuild bit for
+// bit.  Atoms moved since the last full build (binning) are scanned
+// explicitly, as their cells are stale.
+constexpr int kPatchEdits = 32;      // edits one row can take in one update
+constexpr int kPatchRows = 4096;     // moved atoms one update can take
+constexpr int kPatchStale = 2048;    // atoms moved since the binning before a full build is asked for
+struct ListPatch {
+    int *moved_flag, *moved_list, *moved_cnt;   // moved since the last update (flag, list, count)
+    int *stale_flag, *stale_list, *stale_cnt;   // moved since the last full build
+    int *edit_cnt, *edit_val;                    // [n], [n * kPatchEdits]: +(a+1) insert a, -(a+1) delete a
+    int *touched, *touched_cnt;                  // rows with edits
+    int *newrow, *newcnt, *newhalf;              // [kPatchRows * cap], [kPatchRows] rebuilt rows of moved atoms
+    unsigned *ticket;                            // blocks of the apply kernel finished
+};
+// Queues the two patch kernels; a row outgrowing the capacity, more edits
+// or moved atoms than one update takes, or kPatchStale atoms moved since the
+// binning set s.overflow (the next Metropolis chunk then bails out and the
+// host runs a full build).  Returns false when this system cannot be patched
+// (too many atoms for the LDS bitmaps, rows too long for the apply kernel).
+bool list_patch_supported(const DevSys &s);
+void launch_list_patch(const DevSys &s, const CellGrid &g, const ListPatch &p, hipStream_t st);
+// zeroes the patch state (after a full build)
+void launch_list_patch_reset(const DevSys &s, const ListPatch &p, hipStream_t st);
 // bonded.hip
 void launch_bonded(const DevSys &s, const DevTerms &t, double *partial, int nblocks, hipStream_t st);
 int bonded_blocks(const DevTerms &t);
@@ -203,6 +229,10 @@ struct MCRunArgs {
     int *bail;             // device scalar (INT_MAX: none), or null
     // evaluate several proposals per round (no force-bias move type)
     int spec;
+    // incremental list updates: every committed atom is recorded (null: no
+    // record; the list is rebuilt in full)
+    ListPatch patch;
+    int record_moves;
 };
 void launch_mc_chunk(const DevSys &s, const DevTerms &t, const DevMoves &mv, const MCRunArgs &a,
                      hipStream_t st);
@@ -210,23 +240,30 @@ void launch_mc_chunk(const DevSys &s, const DevTerms &t, const DevMoves &mv, con
 // round evaluates the proposal g moves ahead on the assumption that the
 // moves before it are rejected; the rounds resolve in chain order.
 constexpr int kMaxWide = 32;        // workgroups of one launch
-constexpr int kDefaultWide = 8;     // workgroups unless MCX_WIDE says otherwise
+constexpr int kDefaultWide = 8;     // proposals per round unless MCX_WIDE says otherwise
+constexpr int kDefaultSplit = 2;    // workgroups per proposal unless MCX_WIDE_SPLIT says otherwise
 constexpr int kWideRing = 512;      // draws generated ahead (power of two)
 // the draw state of a wide run: the common layout (kRngWords words, last
 // word = 2) followed by the count of drawn-but-unused draws and the draws
 constexpr int kRngWordsWide = kRngWords + 1 + 2 * kWideRing;
 struct WideArgs {
     int nwg;                 // workgroups of the launch (<= kMaxWide)
-    unsigned *sync;          // [1] round-barrier arrivals, zero at launch
+    unsigned *sync;          // [1] round-barrier arrivals, zero at launch (the last workgroup to finish clears it)
+    unsigned *exit_ctr;      // [1] workgroups finished, zero at launch (likewise)
+    unsigned *arrive;        // [1] workgroups arrived at the start-of-launch rendezvous (likewise)
+    int *go;                 // [1] rendezvous decision: 0 open, 1 all resident, 2 abandoned (likewise)
+    long long rdv_ticks;     // give-up time of the rendezvous (s_memrealtime ticks)
     int *err;                // [1] set when a round barrier gave up waiting
     long long spin_ticks;    // give-up time of a barrier wait (s_memrealtime ticks)
-    double *bde;             // [kMaxWide] dE of each round slot
+    int split;               // workgroups per proposal (its pair wavefronts shared out), nwg a multiple
+    double *bred;            // [kMaxWide * 16 * 3] wavefront sums (elec, vdw, bonded) of each proposal
     int4 *bmeta;             // [kMaxWide] (type index, first atom, moved atoms, instance)
     double *bxyz;            // [kMaxWide * kMaxMoveAtoms * 3] trial coordinates
     int4 *bip;               // [kMaxWide * kMaxMoveAtoms] trial fixed-point mirror
 };
-// Cooperative launch of nwg workgroups (all resident at once); returns the
-// launch status.
+// Launch of nwg workgroups.  The kernel starts with a rendezvous: unless
+// every workgroup is resident within rdv_ticks, it changes nothing, sets
+// *err = 3 and the bail-out step; the host then runs the chunk again.
 hipError_t launch_mc_wide(const DevSys &s, const DevTerms &t, const DevMoves &mv, const MCRunArgs &a,
                           const WideArgs &w, hipStream_t st);
 // workgroups of k_mc_wide that can be resident at once on the device

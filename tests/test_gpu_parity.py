@@ -326,9 +326,10 @@ def test_mc_replay_small_box():
 
 KERNELS = {"one_at_a_time": {"MCX_SPECULATE": "0"},
            "read_ahead": {"MCX_SPECULATE": "1", "MCX_WIDE": "0"},
-           "wide16": {"MCX_SPECULATE": "1", "MCX_WIDE": "16"},
-           "wide5": {"MCX_SPECULATE": "1", "MCX_WIDE": "5"},
-           "wide32": {"MCX_SPECULATE": "1", "MCX_WIDE": "32"}}
+           "wide_default": {"MCX_SPECULATE": "1"},
+           "wide5_split1": {"MCX_SPECULATE": "1", "MCX_WIDE": "5", "MCX_WIDE_SPLIT": "1"},
+           "wide4_split3": {"MCX_SPECULATE": "1", "MCX_WIDE": "4", "MCX_WIDE_SPLIT": "3"},
+           "wide16_split2": {"MCX_SPECULATE": "1", "MCX_WIDE": "16", "MCX_WIDE_SPLIT": "2"}}
 
 
 def _chain_case(case):
@@ -354,9 +355,10 @@ def test_mc_chain_kernels_same_chain(case, monkeypatch):
     """Move sets without force-bias moves run a speculative kernel: the
     read-ahead kernel (one workgroup, the next proposal drawn and loaded
     while the current move's pairs are evaluated; MCX_WIDE=0) or, by
-    default, the wide kernel (MCX_WIDE workgroups, each evaluating the
-    proposal some moves ahead on the assumption that the moves before it
-    are rejected, resolved in chain order).  MCX_SPECULATE=0 selects the
+    default, the wide kernel (MCX_WIDE proposals per round, each some moves
+    ahead on the assumption that the moves before it are rejected, its
+    pairs shared out over MCX_WIDE_SPLIT workgroups, resolved in chain
+    order).  MCX_SPECULATE=0 selects the
     one-proposal-at-a-time kernel.  All give the same chain bit for bit: the
     same dE, decisions, dmax and coordinates, through list rebuilds, chunk
     boundaries (the draw state carried over), dmax adaptation steps (which
@@ -370,7 +372,7 @@ def test_mc_chain_kernels_same_chain(case, monkeypatch):
     n = 3000
     out = {}
     for name, env in KERNELS.items():
-        for k in ("MCX_SPECULATE", "MCX_WIDE"):
+        for k in ("MCX_SPECULATE", "MCX_WIDE", "MCX_WIDE_SPLIT"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
             monkeypatch.setenv(k, v)
@@ -550,3 +552,51 @@ def test_gradient_vs_fp64_and_fd():
         assert np.max(np.abs(g_ - o_)) <= 1e-4 * scale
     for g_, o_ in ((ge, oe), (gs, os_), (gq, oq)):
         assert np.max(np.abs(g_ - o_)) <= 1e-4 * max(np.max(np.abs(o_)), 1e-12)
+
+
+@pytest.mark.parametrize("case", ["water1536", "bonded_mix", "water_small_box", "mixed_small_frozen", "water12288"])
+def test_incremental_list_equals_full_rebuild(case, monkeypatch):
+    """Scheduled rebuilds inside mcx_mc_run are incremental updates (rows of
+    the atoms moved since the last update rebuilt, the other rows patched
+    with sorted edits); MCX_LIST_PATCH=0 rebuilds in full.  Both give the
+    same chain bit for bit, and the list after the run equals the FP64
+    oracle's list at the final coordinates (bit-exact: the same pairs in the
+    same order).  water_small_box has a single cell; mixed_small_frozen has
+    frozen atoms; water12288 runs long enough for more than 2,048 atoms to
+    move since the binning, which asks for a full build in between."""
+    import copy
+    if case == "water1536":
+        s = systems.water_box(seed=3, nside=8, L=24.85)[0]
+    elif case == "bonded_mix":
+        s = systems.bonded_mix(seed=7)[0]
+    elif case == "water_small_box":
+        s = systems.water_box(seed=21, nside=5, L=(90 / 0.0334) ** (1 / 3), nmol=90, temp=290.0, tag=300)[0]
+    elif case == "mixed_small_frozen":
+        s = systems.mixed_system(seed=3, nside=8, L=24.85, nbead=60)[0]
+        s.frozen[::7] = 1
+    else:
+        s = systems.water_box(seed=2)[0]
+    n = 12000 if case == "water12288" else 3000
+    out = {}
+    for patch in ("0", "1"):
+        monkeypatch.setenv("MCX_LIST_PATCH", patch)
+        sc = copy.deepcopy(s)
+        ff = mcx.FField(sc)
+        print("case", case, "patch", patch, flush=True)
+        res = ff.h.mc_run(sc.moves, n, 4242, list_every=50)
+        out[patch] = (res, ff.get_coords(), ff.h.get_list())
+    (r0, c0, l0), (r1, c1, l1) = out["0"], out["1"]
+    assert 0 < int(r0["acc"].sum()) < n
+    for key in ("acc", "de", "mvtyp", "mvinst", "trials", "accepted", "dmax"):
+        assert np.array_equal(r0[key], r1[key]), key
+    assert r0["etot"] == r1["etot"]
+    for a, b in zip(c0, c1):
+        assert np.array_equal(a, b)
+    assert np.array_equal(l0[0], l1[0]) and np.array_equal(l0[1], l1[1])
+    # n is a multiple of 50: the list was brought up to date at the end
+    ref = s.copy()
+    ref.x[:], ref.y[:], ref.z[:] = c1
+    o = Oracle(ref)
+    o.update_list()
+    cnt_o, off_o, flat_o = o.list_csr()
+    assert np.array_equal(cnt_o, l1[0]) and np.array_equal(flat_o, l1[1])
