@@ -25,12 +25,18 @@ the adjoint kernel's time on that box and a 64-configuration batched
 adjoint call -> "force_bias".
 
 Also measured in the same run (HIP events on the library's own stream):
-  * the O(N^2) pair kernel (full min-image LJ+Coulomb energy of the box):
-    pair-evals/s and FP32 fraction of peak -> "roofline";
-  * the list build: algorithmic HBM bytes/s -> "list_build";
-  * the reference's own CPU code (oracle/_ref/libmcref.so, compiled from the
-    mc_cpp sources) on this host: moves/s of its two-getE-per-move loop
-    on a bounded sample -> "cpu_baseline".
+  * "roofline": the Metropolis kernel of the headline loop -- algorithmic
+    FLOP of a move's dE (moved atoms x full-row candidates x 2 x (17 + 23
+    f_in)) x moves per launch / mean launch time, every launch of the timed
+    region bracketed by HIP events (MCX_KERNEL_EVENTS=1);
+  * "secondary_rooflines": the O(N^2) pair kernel, the switched list
+    energy kernel (the drop-in getE), the full list build, the in-run list
+    update and the batched sweep kernel;
+  * "getE_end_to_end": host coordinate upload + energy, as the drop-in
+    FField_MCX::getE does it, beside the reference's getE on the CPU;
+  * "cpu_baseline": the reference's own CPU loop (oracle/_ref/libmcref.so,
+    compiled from the mc_cpp sources) on one core, the FP64 restatement on
+    all cores and in its CPU dE mode, bounded samples on this host.
 """
 from __future__ import annotations
 
@@ -45,25 +51,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
-FLOP_FULL_PAIR = 32.0          # per pair, FMA = 2 (the constant frozen in csrc/mcx_internal.h)
+# algorithmic FLOP per unit, FMA = 2 (the constants frozen in csrc/mcx_internal.h)
+FLOP_FULL_PAIR = 32.0          # full O(N^2) pair
+FLOP_LIST_ENTRY = 17.0         # switched list entry (distance, min image, r^2)
+FLOP_LIST_INCUT = 23.0         # + per entry inside the cutoff
 COPY_BYTES = 256 << 20         # calibration copy of the counter runs
-
-
-def cpu_reference_baseline(s, fp, nmoves, seed):
-    """Times the reference's MC_metropolis::run (two full getE per move) on
-    a bounded sample on one core; returns (moves_per_s, seconds)."""
-    from oracle import oracle
-    from mc_cpp_amd import systems
-    if not oracle.ref_available():
-        return None
-    d = tempfile.mkdtemp(prefix="mcx_bench_")
-    cor, ff = systems.save_mdbas(os.path.join(d, "water"), s, fp)
-    r = oracle.Reference(cor, ff, s)
-    t0 = time.perf_counter()
-    r.mc_run(s.moves, nmoves, seed)
-    dt = time.perf_counter() - t0
-    return nmoves / dt, dt
 
 
 def batched_replicas(args):
@@ -88,6 +82,13 @@ def batched_replicas(args):
         nm += s.natom // 3
     pmps = nm / tm
     natoms = sum(s.natom for s, _ in reps)
+    # algorithmic FLOP of one sweep (one move per molecule in every replica)
+    flop_sweep = 0.0
+    for (s, _), h in zip(reps, handles):
+        npairs, nin, row_mean = list_stats(h, s)
+        flop_sweep += (s.natom // 3) * move_flop(3.0, row_mean, nin / max(npairs, 1))
+    ms_sweep = out["seconds"] / args.batch_sweeps * 1e3
+    tf = flop_sweep / (ms_sweep * 1e-3) / 1e12
     return {"workload": "%d water boxes, %d-%d molecules (%d atoms), T ladder 280-400 K, exchange every 100 sweeps, "
                         "residue TRN 0.3 A + ROT 15 deg, sweep = one move per molecule"
                         % (nrep, min(s.natom for s, _ in reps) // 3, max(s.natom for s, _ in reps) // 3, natoms),
@@ -99,7 +100,11 @@ def batched_replicas(args):
             "launch_shape": {"threads": threads, "tiles_in_flight": unroll, "tuned_ms_per_sweep": ms_tuned},
             "plain_loop_moves_per_s": pmps,
             "plain_loop": "each replica one sweep through the single-chain device loop (mcx_mc_run, list every 50)",
-            "speedup_vs_plain": bmps / pmps}
+            "speedup_vs_plain": bmps / pmps,
+            "roofline": {"bound": "valu", "kernel": "batched sweep kernel (one persistent launch per sweep, "
+                                                    "FP64 pair change; time per sweep from the graph replays)",
+                         "achieved": tf, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": tf / PEAK_FP64_TFLOPS,
+                         "flop_per_launch": flop_sweep, "ms_per_launch": ms_sweep, "traffic": None}}
 
 
 def force_bias_vs_translation(args):
@@ -169,33 +174,67 @@ def large_mixed(args):
     return out
 
 
-def kernel_probe():
+def list_stats(h, s):
+    """(list entries, entries inside the cutoff, mean full-row length) of
+    the handle's current pair list, from the list itself and the
+    coordinates (min-image, the run's box)."""
+    import numpy as np
+    cnt, flat = h.get_list()
+    x, y, z = h.get_coords()
+    i = np.repeat(np.arange(s.natom, dtype=np.int64), cnt)
+    j = flat.astype(np.int64)
+    r2 = np.zeros(len(j))
+    for c, L in zip((x, y, z), s.box):
+        d = c[j] - c[i]
+        if s.pbtype != 0:
+            d -= L * np.rint(d / L)
+        r2 += d * d
+    nin = int((r2 <= s.ctoff * s.ctoff).sum())
+    return int(len(j)), nin, 2.0 * len(j) / s.natom
+
+
+def move_flop(atoms_per_move, row_mean, f_in):
+    """Algorithmic FLOP of one move's dE: every moved atom's full-row
+    candidates, old and new position, 17 per candidate + 23 in cutoff."""
+    return atoms_per_move * row_mean * 2.0 * (FLOP_LIST_ENTRY + FLOP_LIST_INCUT * f_in)
+
+
+def kernel_probe(args):
     """Fixed kernel sequence for the HBM-traffic counter runs
-    (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE): 20 launches of the O(N^2)
-    pair kernel on the 12,288-atom box, then 5 launches of a plain device
-    copy of COPY_BYTES (torch's elementwise copy of float32), whose counter
-    readings calibrate the counters' units."""
+    (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE): the headline device loop for
+    --probe-moves moves (Metropolis kernel launches of 50 moves with their
+    list updates), 20 launches of the O(N^2) pair kernel and of the
+    switched list-energy kernel, 5 list builds, then 5 launches of a plain
+    device copy of COPY_BYTES (torch's elementwise copy of float32), whose
+    counter readings calibrate the counters' units."""
     import torch
     from mc_cpp_amd import mcx, systems
     s, _ = systems.water_box(seed=2)
-    s.cutmode = systems.CUT_FULL
-    h = mcx.from_system(s)
-    h.energy()
-    h.time_kernel(0, 20)
+    h = mcx.FField(s).h
+    h.mc_run(s.moves, 500, 1000, rng_mode=mcx.MC_REPLAY, list_every=50, record=False)
+    h.mc_run(s.moves, args.probe_moves, 2000, rng_mode=mcx.MC_REPLAY, list_every=50, record=False)
+    h.time_kernel(1, 20)
+    h.time_kernel(2, 5)
+    sf = s.copy()
+    sf.cutmode = systems.CUT_FULL
+    hf = mcx.from_system(sf)
+    hf.energy()
+    hf.time_kernel(0, 20)
     src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
     torch.cuda.synchronize()
     for _ in range(5):
         dst.copy_(src)
     torch.cuda.synchronize()
-    print(json.dumps({"probe": "k_full_energy x21 (1 warm-up), copy x5", "copy_bytes": COPY_BYTES,
-                      "atoms": s.natom}), flush=True)
+    print(json.dumps({"probe": "mc_run %d+500 moves, k_switch_energy x21, list build x6, k_full_energy x21 "
+                               "(1 warm-up each), copy x5" % args.probe_moves,
+                      "copy_bytes": COPY_BYTES, "atoms": s.natom}), flush=True)
 
 
-def traffic_from_profile():
-    """HBM bytes per launch of k_full_energy from the committed counter
-    summary (a line 'traffic_bytes_per_launch: <int>'), or None."""
-    path = os.path.join(ROOT, "profiles", "k_full_energy_water12288_pmc.md")
+def traffic_from_profile(name):
+    """HBM bytes per launch of a kernel from its committed counter summary
+    (a line 'traffic_bytes_per_launch: <int>' in profiles/<name>), or None."""
+    path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
             for line in f:
@@ -206,14 +245,106 @@ def traffic_from_profile():
     return None
 
 
+def host_info():
+    nproc = len(os.sched_getaffinity(0))
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": nproc, "os_cpu_count": os.cpu_count(), "model": model}
+
+
+def cpu_baselines(s, fp, args):
+    """CPU timings on this host, bounded samples of the headline workload:
+      * the reference's own MC_metropolis::run (oracle/_ref/libmcref.so, two
+        full switched getE per move, list rebuilt every 50 moves), one
+        thread -- the reference's energy loops are serial;
+      * the FP64 restatement (oracle) in the same compat mode on all cores
+        (rows of the pair loops dealt out by OpenMP);
+      * the restatement's CPU dE mode (only the pairs of the moved atoms,
+        old and new, same decisions), one thread and all cores;
+      * one end-to-end reference getE() (src/mc_cpp.cpp:169-201 times this
+        call).
+    Returns the cpu_baseline object."""
+    import ctypes
+    from oracle import oracle
+    from mc_cpp_amd import systems
+    info = host_info()
+    allc = max(1, min(info["nproc"], args.cpu_threads))
+    modes = {}
+    out = {"host": info}
+    gomp = None
+    try:
+        gomp = ctypes.CDLL("libgomp.so.1")
+    except OSError:
+        pass
+
+    def oracle_run(mode, nmoves, threads):
+        sc = s.copy()
+        o = oracle.Oracle(sc)
+        o.update_list(threads)
+        oracle.set_energy_threads(threads)
+        try:
+            t0 = time.perf_counter()
+            o.mc_run(nmoves, 7, mode=mode, list_every=50, nthreads=threads)
+            dt = time.perf_counter() - t0
+        finally:
+            oracle.set_energy_threads(1)
+        return {"moves_per_s": nmoves / dt, "moves": nmoves, "seconds": dt, "threads": threads}
+
+    ref = None
+    if oracle.ref_available():
+        d = tempfile.mkdtemp(prefix="mcx_bench_")
+        cor, ff = systems.save_mdbas(os.path.join(d, "water"), s.copy(), fp)
+        if gomp is not None:
+            gomp.omp_set_num_threads(1)
+        r = oracle.Reference(cor, ff, s.copy())
+        t0 = time.perf_counter()
+        r.getE()
+        ge = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        r.mc_run(s.moves, args.cpu_moves, 7)
+        dt = time.perf_counter() - t0
+        ref = {"moves_per_s": args.cpu_moves / dt, "moves": args.cpu_moves, "seconds": dt, "threads": 1}
+        modes["reference_compat_1thread"] = ref
+        out["reference_getE_ms"] = ge * 1e3
+        if gomp is not None:
+            gomp.omp_set_num_threads(info["nproc"])
+    modes["port_compat_allcores"] = oracle_run(0, args.cpu_moves, allc)
+    modes["port_dE_1thread"] = oracle_run(1, args.cpu_de_moves, 1)
+    modes["port_dE_allcores"] = oracle_run(1, args.cpu_de_moves, allc)
+    out["modes"] = modes
+    if ref is not None:
+        out.update({"value": ref["moves_per_s"], "unit": "moves/s", "cores": 1, "kind": "reference",
+                    "sample": "%d moves of the reference's MC_metropolis::run on the 12,288-atom water box (two "
+                              "full switched getE per move, list rebuilt every 50 moves), %.1f s on one core; "
+                              "'modes' adds the FP64 restatement on %d cores and its CPU dE mode"
+                              % (ref["moves"], ref["seconds"], allc)})
+    else:
+        m = modes["port_compat_allcores"]
+        out.update({"value": m["moves_per_s"], "unit": "moves/s", "cores": allc, "kind": "port",
+                    "sample": "%d moves of the FP64 restatement of MC_metropolis::run (two full switched "
+                              "energies per move), %.1f s on %d cores" % (m["moves"], m["seconds"], allc)})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--moves", type=int, default=20000, help="Metropolis moves per step")
-    ap.add_argument("--cpu-moves", type=int, default=24, help="moves of the reference CPU loop to time")
+    ap.add_argument("--cpu-moves", type=int, default=200, help="moves of the CPU compat-mode runs")
+    ap.add_argument("--cpu-de-moves", type=int, default=200, help="moves of the CPU dE-mode runs")
+    ap.add_argument("--cpu-threads", type=int, default=16, help="at most this many threads for 'all cores'")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--event-every", type=int, default=8,
+                    help="bracket every k-th kernel launch of the loop with HIP events (0: none)")
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--batch-sweeps", type=int, default=100)
     ap.add_argument("--batch-warmup", type=int, default=1)
@@ -221,16 +352,19 @@ def main():
     ap.add_argument("--fb-moves", type=int, default=20000, help="moves per force-bias / translation run")
     ap.add_argument("--no-fbias", action="store_true")
     ap.add_argument("--probe", action="store_true", help="kernel sequence for the HBM counter runs only")
+    ap.add_argument("--probe-moves", type=int, default=2000)
     ap.add_argument("--large-moves", type=int, default=10000, help="moves per run on the ~100k-atom system")
     ap.add_argument("--no-large", action="store_true")
     args = ap.parse_args()
     if args.probe:
-        kernel_probe()
+        kernel_probe(args)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the Metropolis chain is sequential and this build has no "
               "multi-GPU path; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.event_every > 0:
+        os.environ["MCX_KERNEL_EVENTS"] = str(args.event_every)
 
     import numpy as np
     from mc_cpp_amd import mcx, systems
@@ -252,13 +386,49 @@ def main():
     t0 = time.perf_counter()
     acc = 0
     nlist = 0
+    ev = {"chain_kernel_ms": 0.0, "chain_launches": 0, "list_kernel_ms": 0.0, "list_launches": 0}
     for k in range(args.steps):
         r = h.mc_run(s.moves, args.moves, 2000 + k, rng_mode=mcx.MC_REPLAY, list_every=50, record=False)
         acc += int(r["accepted"].sum())
         nlist += int(r["list_updates"])
+        for key in ev:
+            ev[key] += r[key]
     sync()
     dt = time.perf_counter() - t0
-    moves_per_s = args.steps * args.moves / dt
+    nmoves = args.steps * args.moves
+    moves_per_s = nmoves / dt
+
+    # ---- algorithmic work of a move, from the list the run ended with ----
+    npairs, nin, row_mean = list_stats(h, s)
+    f_in = nin / npairs
+    atoms_per_move = s.natom / len(np.unique(s.resid))        # residue moves of 3-atom waters
+    flop_move = move_flop(atoms_per_move, row_mean, f_in)
+    chain = None
+    if ev["chain_launches"]:
+        ms_launch = ev["chain_kernel_ms"] / ev["chain_launches"]
+        moves_launch = args.moves / -(-args.moves // 50)          # a launch runs the moves between list updates
+        achieved = flop_move * moves_launch / (ms_launch * 1e-3) / 1e12
+        chain = {"bound": "valu", "kernel": "k_mc_wide (device Metropolis loop: %d moves per launch, "
+                                            "FP64 pair change)" % round(moves_launch),
+                 "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                 "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic_from_profile("k_mc_wide_water12288_pmc.md"),
+                 "flop_per_move": flop_move, "flop_per_launch": flop_move * moves_launch,
+                 "ms_per_launch": ms_launch, "launches_timed": ev["chain_launches"],
+                 "us_per_move_in_kernel": ms_launch * 1e3 / moves_launch,
+                 "frac_of_fp32_peak": achieved / PEAK_FP32_TFLOPS,
+                 "work": "per move: %.2f moved atoms x %.1f full-row candidates x 2 (old, new) x "
+                         "(17 + 23 x f_in), f_in = %.4f in-cutoff fraction of the list" % (atoms_per_move, row_mean, f_in)}
+
+    # ---- end-to-end getE as the drop-in FField does it: upload + energy ----
+    x, y, z = h.get_coords()
+    h.set_coords(x, y, z)
+    h.energy()
+    reps_e = 50
+    t0 = time.perf_counter()
+    for _ in range(reps_e):
+        h.set_coords(x, y, z)
+        h.energy()
+    getE_ms = (time.perf_counter() - t0) / reps_e * 1e3
 
     # ---- pair kernel: full O(N^2) energy, HIP events on the library stream ----
     n = s.natom
@@ -274,24 +444,52 @@ def main():
 
     # ---- switched list energy kernel and list build ----
     ms_sw = h.time_kernel(1, reps)
+    flop_sw = FLOP_LIST_ENTRY * npairs + FLOP_LIST_INCUT * nin
+    tf_sw = flop_sw / (ms_sw * 1e-3) / 1e12
     ms_list = h.time_kernel(2, 20)
-    npairs = h.update_list()
-    list_bytes = 24 * n + 4 * (n + 1) + 4 * npairs
+    npairs_b = h.update_list()
+    list_bytes = 24 * n + 4 * (n + 1) + 4 * npairs_b
     list_gbs = list_bytes / (ms_list * 1e-3) / 1e9
+    patch = None
+    if ev["list_launches"]:
+        ms_p = ev["list_kernel_ms"] / ev["list_launches"]
+        patch = {"bound": "hbm", "kernel": "k_patch_rows + k_patch_apply (in-run list update every 50 moves; "
+                                           "the same list as a full rebuild)",
+                 "achieved": list_bytes / (ms_p * 1e-3) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                 "frac": list_bytes / (ms_p * 1e-3) / 1e9 / PEAK_HBM_GBS, "ms_per_update": ms_p,
+                 "updates_timed": ev["list_launches"], "bytes_per_update": list_bytes}
 
     batch = None if args.no_batch else batched_replicas(args)
     fbias = None if args.no_fbias else force_bias_vs_translation(args)
     large = None if args.no_large else large_mixed(args)
+    cpu = None if args.no_cpu else cpu_baselines(s_cpu, fp, args)
+    if cpu is not None:
+        cpu["gpu_over_cpu"] = moves_per_s / cpu["value"]
 
-    cpu = None
-    if not args.no_cpu:
-        res = cpu_reference_baseline(s_cpu, fp, args.cpu_moves, 7)
-        if res is not None:
-            mps, secs = res
-            cpu = {"value": mps, "unit": "moves/s", "cores": 1, "kind": "reference",
-                   "sample": "%d moves of MC_metropolis::run on this box (two full switched getE per move, "
-                             "single-threaded energy loops as the reference builds them), %.1f s"
-                             % (args.cpu_moves, secs)}
+    secondary = {
+        "k_full_energy": {"bound": "valu", "kernel": "k_full_energy (O(N^2) min-image LJ+Coulomb, 12,288 atoms)",
+                          "achieved": tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                          "frac": tflops / PEAK_FP32_TFLOPS,
+                          "traffic": traffic_from_profile("k_full_energy_water12288_pmc.md"),
+                          "peak_without_packed_fp32": PEAK_FP32_TFLOPS / 2,
+                          "frac_without_packed_fp32": tflops / (PEAK_FP32_TFLOPS / 2),
+                          "flop_per_launch": flop, "ms_per_launch": ms_full,
+                          "pair_evals_per_s": pairs / (ms_full * 1e-3)},
+        "k_switch_energy": {"bound": "valu", "kernel": "k_switch_energy (switched list energy, the getE of "
+                                                       "the drop-in path)",
+                            "achieved": tf_sw, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                            "frac": tf_sw / PEAK_FP32_TFLOPS,
+                            "traffic": traffic_from_profile("k_switch_energy_water12288_pmc.md"),
+                            "flop_per_launch": flop_sw, "ms_per_launch": ms_sw, "list_entries": npairs,
+                            "in_cutoff": nin},
+        "list_build": {"bound": "hbm", "kernel": "full list build (bin, rows, compaction)",
+                       "achieved": list_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                       "frac": list_gbs / PEAK_HBM_GBS, "bytes_per_launch": list_bytes, "ms_per_launch": ms_list,
+                       "pairs": npairs_b},
+        "list_update_in_run": patch,
+    }
+    if batch is not None:
+        secondary["k_batch_sweep"] = batch.pop("roofline", None)
 
     out = {
         "metric": "MC moves/s, pair-evals/s (%FP32 peak) on 12k-atom water; batched vs plain",
@@ -304,25 +502,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 pair math, f64 Coulomb distance chain and sums",
+        "dtype": "f64",
         "data": "synthetic (seeded Philox water lattice, random orientations)",
         "config": {"workload": "water_box_12288: 4096 rigid TIP3P-like waters, L=49.7 A, switch 10/12 A, "
                                "skin 2 A, T=300 K, residue TRN 0.3 A + ROT 15 deg, plain single-chain "
-                               "device Metropolis loop, list rebuilt every 50 moves",
-                   "moves_per_step": args.moves, "acceptance": acc / (args.steps * args.moves),
-                   "list_rebuilds": nlist},
+                               "device Metropolis loop (MT19937 replay of the reference's draws), list "
+                               "updated every 50 moves",
+                   "moves_per_step": args.moves, "acceptance": acc / nmoves,
+                   "list_rebuilds": nlist, "kernel_events_every": args.event_every},
+        "roofline": chain,
+        "secondary_rooflines": secondary,
         "pair_evals_per_s": pairs / (ms_full * 1e-3),
         "full_energy_ms": ms_full,
         "switch_energy_ms": ms_sw,
-        "roofline": {"bound": "valu", "kernel": "k_full_energy (O(N^2) min-image LJ+Coulomb, 12,288 atoms)",
-                     "achieved": tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": tflops / PEAK_FP32_TFLOPS, "traffic": traffic_from_profile(),
-                     "peak_without_packed_fp32": PEAK_FP32_TFLOPS / 2,
-                     "frac_without_packed_fp32": tflops / (PEAK_FP32_TFLOPS / 2),
-                     "flop_per_launch": flop, "ms_per_launch": ms_full},
-        "list_build": {"bound": "hbm", "achieved": list_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                       "frac": list_gbs / PEAK_HBM_GBS, "bytes_per_launch": list_bytes, "ms_per_launch": ms_list,
-                       "pairs": npairs},
+        "getE_end_to_end": {"ms": getE_ms, "what": "mcx_set_coords from host arrays + mcx_energy (the drop-in "
+                                                   "FField_MCX::getE), wall clock, PCIe included",
+                            "reference_cpu_ms": cpu.get("reference_getE_ms") if cpu else None},
         "batched": batch,
         "force_bias": fbias,
         "large_mixed": large,

@@ -166,6 +166,14 @@ typedef struct mcx_mc_stats {
     double dmax[8];        /* final dmax per move type */
     long long list_updates;
     double seconds;        /* wall time of the device loop */
+    /* With the environment variable MCX_KERNEL_EVENTS=k (k >= 1): the
+     * device time of every k-th Metropolis kernel launch of the run and of
+     * every k-th in-run list update, each bracketed by HIP events on the
+     * handle's stream, and how many were timed (0 else). */
+    double chain_kernel_ms;
+    long long chain_launches;
+    double list_kernel_ms;
+    long long list_launches;
 } mcx_mc_stats;
 
 #define MCX_LIST_AUTO  (-1) /* list_every: rebuild when an accepted move carries an atom beyond half the skin */

@@ -1048,12 +1048,36 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
             wa.split = split;
             wa.bred = bred_d.p; wa.bmeta = bmeta_d.p; wa.bxyz = bxyz_d.p; wa.bip = bip_d.p;
         }
+        // MCX_KERNEL_EVENTS=k (k >= 1): every k-th chunk and every k-th
+        // in-run list update between a pair of HIP events (pairs from the
+        // handle's pool), summed at the end
+        const char *kev = std::getenv("MCX_KERNEL_EVENTS");
+        const int ev_every = kev ? std::max(0, std::atoi(kev)) : 0;
+        std::vector<std::pair<int, int>> ev_used;     // (pool index of the first event, 0 chain / 1 list)
+        long long ev_seen[2] = {0, 0};
+        auto ev_begin = [&](int what) -> int {
+            if (!ev_every || ev_seen[what]++ % ev_every != 0) return -1;
+            const size_t k = 2 * ev_used.size();
+            while (h->ev_pool.size() < k + 2) {
+                hipEvent_t e;
+                HIPCHK(hipEventCreate(&e));
+                h->ev_pool.push_back(e);
+            }
+            ev_used.emplace_back((int)k, what);
+            HIPCHK(hipEventRecord(h->ev_pool[k], h->stream));
+            return (int)k;
+        };
+        auto ev_end = [&](int k) {
+            if (k >= 0) HIPCHK(hipEventRecord(h->ev_pool[k + 1], h->stream));
+        };
         auto launch_chunk = [&]() {
+            const int ev = ev_begin(0);
             if (nwg) {
                 HIPCHK(launch_mc_wide(h->sys(), h->dterms(), mv, a, wa, h->stream));
             } else {
                 launch_mc_chunk(h->sys(), h->dterms(), mv, a, h->stream);
             }
+            ev_end(ev);
         };
         // true: a launch found its workgroups not resident together and
         // changed nothing (it set the bail-out step); it is run again
@@ -1145,12 +1169,21 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
             a.bail = bail_d.p;
         }
         int since_check = 0;
+        // the incremental updates test every atom moved since the last full
+        // build from an explicit list, which a full build empties; when a
+        // check finds it past half its limit, the next scheduled update is a
+        // full build (before the limit makes a patch give up and the chunks
+        // after it bail out)
+        bool full_next = false;
         auto check_bail = [&]() {
             check_wide();
             if (!a.bail) return;
-            int b = kNone;
+            int b = kNone, nst = 0;
             HIPCHK(hipMemcpyAsync(&b, bail_d.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+            if (use_patch)
+                HIPCHK(hipMemcpyAsync(&nst, h->patch().stale_cnt, sizeof(int), hipMemcpyDeviceToHost, h->stream));
             HIPCHK(hipStreamSynchronize(h->stream));
+            if (nst > kPatchStale / 2) full_next = true;
             if (b != kNone) {
                 done = b;
                 if (rebuilds) {
@@ -1171,13 +1204,17 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
             HIPCHK(hipGetLastError());
             done += chunk;
             if (rebuilds && done % list_every == 0) {
-                if (use_patch) {
+                const int ev = ev_begin(1);
+                if (use_patch && !full_next) {
                     h->p_newrow.alloc((size_t)std::min(h->n, kPatchRows) * h->cap);   // grows with the capacity
                     launch_list_patch(h->sys(), h->grid, h->patch(), h->stream);
                 } else {
                     launch_list_build(h->sys(), h->grid, h->stream);
+                    if (use_patch) launch_list_patch_reset(h->sys(), h->patch(), h->stream);
+                    full_next = false;
                 }
                 HIPCHK(hipGetLastError());
+                ev_end(ev);
                 nlist++;
             }
             const bool frame = save && done % save == 0;
@@ -1210,6 +1247,12 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
             HIPCHK(hipMemcpy(stats->dmax, dmax.p, ntypes * sizeof(double), hipMemcpyDeviceToHost));
             stats->list_updates = nlist;
             stats->seconds = secs;
+            for (const auto &u : ev_used) {
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, h->ev_pool[u.first], h->ev_pool[u.first + 1]));
+                if (u.second == 0) { stats->chain_kernel_ms += ms; stats->chain_launches++; }
+                else { stats->list_kernel_ms += ms; stats->list_launches++; }
+            }
         }
         sync_host_coords(h);
         if (acc_out && nsteps) HIPCHK(hipMemcpy(acc_out, acc_d.p, nsteps, hipMemcpyDeviceToHost));

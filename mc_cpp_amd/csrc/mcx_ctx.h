@@ -166,6 +166,10 @@ struct mcx_ctx {
         return t;
     }
 
+    // timing events of mcx_mc_run's kernel launches (MCX_KERNEL_EVENTS=1),
+    // kept for the next run
+    std::vector<hipEvent_t> ev_pool;
+
     ~mcx_ctx()
     {
         DBuf<double> *d[] = {&x, &y, &z, &q64, &eps64, &sig64, &eps1464, &sig1464, &mass64, &partial, &result,
@@ -188,6 +192,7 @@ struct mcx_ctx {
         for (auto *b : pb) b->release();
         p_ticket.release();
         for (int k = 0; k < kNumKinds; k++) { t_idx[k].release(); t_type[k].release(); t_par[k].release(); }
+        for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };

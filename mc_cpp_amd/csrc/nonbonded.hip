@@ -11,7 +11,8 @@
 //    free of per-pair branches.
 //  * k_switch_energy: the switched sum over the Verlet list
 //    (computeNonBonded_switch, :336-424).  One wavefront per atom row of
-//    the sorted full list, reading the j > i suffix.
+//    the sorted full list, reading the j > i suffix in chunks whose loads
+//    are all in flight together.
 //
 // Pair arithmetic: the displacement is the wrapping int32 difference of the
 // fixed-point fractional coordinates (the exact minimum image, no rint),
@@ -162,7 +163,14 @@ __global__ __launch_bounds__(kTile) void k_full_energy(DevSys s, const int2 *__r
 // ---------------------------------------------------------------------------
 
 constexpr int kSwWaves = 4;
+constexpr int kSwUnroll = 4;    // 64-entry slices of a row whose loads are in flight together
 
+// One wavefront per row.  The row's j > i suffix is read in chunks of
+// kSwUnroll x 64 entries: first every index of the chunk, then every j
+// atom's position and parameters, then the arithmetic, so that a chunk
+// waits on two memory latencies instead of two per 64 entries.  Each lane
+// keeps its own sums over all rows of the wavefront (q_i and eps_i folded
+// in per row); one wavefront sum at the end.
 template <int PB>
 __global__ __launch_bounds__(64 * kSwWaves) void k_switch_energy(DevSys s, double *__restrict__ partial)
 {
@@ -178,38 +186,52 @@ __global__ __launch_bounds__(64 * kSwWaves) void k_switch_energy(DevSys s, doubl
         const int *__restrict__ row = s.nbr + (size_t)i * s.cap;
         const int k1 = s.cnt[i];
         double c = 0.0, v = 0.0;
-        for (int k = s.half[i] + lane; k < k1; k += 64) {
-            const int j = row[k];
-            const int4 pj = s.ipos[j];
-            const float2 ej = s.esk[j];
-            double dx, dy, dz;
-            disp<PB>(s, pi, pj, xi, yi, zi, j, dx, dy, dz);
-            const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-            const double r2A = r2 * u2;
-            if (r2A <= s.ctoff2) {
-                float ri32;
-                const double ri = mcx_rsqrt(r2, ri32);
-                const float uu = (ei.y + ej.y) * ri32;
-                const float t = uu * uu;
-                const float t3 = t * t * t;
-                const double vj = (double)(ej.x * fmaf(t3, t3, -t3));
-                double sw = 1.0;
-                if (r2A > s.cton2) {
-                    const double a = s.ctoff2 - r2A;
-                    sw = a * a * (s.ctoff2 + 2.0 * r2A - 3.0 * s.cton2) * s.switch2;
+        for (int kb = s.half[i]; kb < k1; kb += 64 * kSwUnroll) {
+            int j[kSwUnroll];
+#pragma unroll
+            for (int u = 0; u < kSwUnroll; u++) {
+                const int k = kb + 64 * u + lane;
+                j[u] = k < k1 ? row[k] : -1;
+            }
+            int4 pj[kSwUnroll];
+            float2 ej[kSwUnroll];
+#pragma unroll
+            for (int u = 0; u < kSwUnroll; u++) {
+                const int jj = j[u] < 0 ? i : j[u];    // a valid address; the pair is not summed
+                pj[u] = s.ipos[jj];
+                ej[u] = s.esk[jj];
+            }
+#pragma unroll
+            for (int u = 0; u < kSwUnroll; u++) {
+                if (j[u] < 0) continue;
+                double dx, dy, dz;
+                disp<PB>(s, pi, pj[u], xi, yi, zi, j[u], dx, dy, dz);
+                const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                const double r2A = r2 * u2;
+                if (r2A <= s.ctoff2) {
+                    float ri32;
+                    const double ri = mcx_rsqrt(r2, ri32);
+                    const float uu = (ei.y + ej[u].y) * ri32;
+                    const float t = uu * uu;
+                    const float t3 = t * t * t;
+                    const double vj = (double)(ej[u].x * fmaf(t3, t3, -t3));
+                    double sw = 1.0;
+                    if (r2A > s.cton2) {
+                        const double a = s.ctoff2 - r2A;
+                        sw = a * a * (s.ctoff2 + 2.0 * r2A - 3.0 * s.cton2) * s.switch2;
+                    }
+                    c = fma(sw * (double)__int_as_float(pj[u].w), ri, c);
+                    v = fma(sw, vj, v);
                 }
-                c = fma(sw * (double)__int_as_float(pj.w), ri, c);
-                v = fma(sw, vj, v);
             }
         }
-        c = wave_sum(c);
-        v = wave_sum(v);
-        ec += (double)__int_as_float(pi.w) * c;
-        ev += (double)ei.x * v;
+        ec = fma((double)__int_as_float(pi.w), c, ec);
+        ev = fma((double)ei.x, v, ev);
     }
+    ec = wave_sum(ec) * s.iunit;
+    ev = wave_sum(ev);
     // lanes hold identical wave sums; keep lane 0's copy only
     if (lane != 0) { ec = 0.0; ev = 0.0; }
-    ec *= s.iunit;
     block_sum2<64 * kSwWaves>(ec, ev);
     if (threadIdx.x == 0) {
         partial[2 * blockIdx.x] = ec;

@@ -47,7 +47,9 @@ class MoveTypeC(C.Structure):
 
 class MCStatsC(C.Structure):
     _fields_ = [("etot", C.c_double), ("trials", C.c_longlong * 8), ("accepted", C.c_longlong * 8),
-                ("dmax", C.c_double * 8), ("list_updates", C.c_longlong), ("seconds", C.c_double)]
+                ("dmax", C.c_double * 8), ("list_updates", C.c_longlong), ("seconds", C.c_double),
+                ("chain_kernel_ms", C.c_double), ("chain_launches", C.c_longlong),
+                ("list_kernel_ms", C.c_double), ("list_launches", C.c_longlong)]
 
 
 class BatchStatsC(C.Structure):
@@ -318,7 +320,9 @@ class Handle:
         self._chk(self.L.mcx_mc_run_traced(self.h, arr, nt, int(nsteps), C.c_uint64(seed), int(rng_mode),
                                            int(list_every), C.byref(st), C.byref(tr)))
         out = dict(etot=st.etot, trials=np.array(st.trials[:nt]), accepted=np.array(st.accepted[:nt]),
-                   dmax=np.array(st.dmax[:nt]), list_updates=st.list_updates, seconds=st.seconds)
+                   dmax=np.array(st.dmax[:nt]), list_updates=st.list_updates, seconds=st.seconds,
+                   chain_kernel_ms=st.chain_kernel_ms, chain_launches=st.chain_launches,
+                   list_kernel_ms=st.list_kernel_ms, list_launches=st.list_launches)
         if record:
             out.update(acc=acc[:nsteps], de=de[:nsteps], mvtyp=ty[:nsteps], mvinst=ins[:nsteps],
                        disp=disp[:nsteps], lnq=lnq[:nsteps])

@@ -376,6 +376,40 @@ static inline void switched_pair(const orc_system *s, double r2, double qi, doub
     *pv = pvdw * sw;
 }
 
+/* Threads of the switched pair loops (orc_nonbonded_switch and the SWITCH
+ * branch of orc_moving_set_energy).  1, the default, keeps the reference's
+ * serial summation order bit for bit.  More deal the rows out dynamically
+ * with per-thread sums, the OpenMP form the reference gives its list build
+ * (src/List_nonBonded.cpp:847-851) and leaves commented out on its energy
+ * loops (src/FField_MDBAS.cpp:211): the totals then differ by rounding
+ * only.  Used to time the CPU on all cores. */
+static int g_energy_threads = 1;
+
+void orc_set_energy_threads(int n) { g_energy_threads = n > 1 ? n : 1; }
+
+/* one row of computeNonBonded_switch :336-424, entries k0..k1-1, summed
+ * into *le / *lv pair by pair */
+static inline void switch_row(const orc_system *s, int i, int k0, int k1, double ctoff2, double cton2,
+                              double switch2, double *le, double *lv)
+{
+    double di[3], dj[3];
+    getc3(s, i, di);
+    const double qi = s->charge[i], epsi = s->eps[i], sigi = s->sig[i];
+    const int *row = s->neighList + (size_t)i * s->cap;
+    for (int k = k0; k < k1; k++) {
+        int j = row[k];
+        getc3(s, j, dj);
+        double r2 = distance2(s, di, dj);
+        if (r2 <= ctoff2) {
+            double pe, pv;
+            switched_pair(s, r2, qi, s->charge[j], epsi, s->eps[j], sigi, s->sig[j],
+                          ctoff2, cton2, switch2, &pe, &pv);
+            *le += pe;
+            *lv += pv;
+        }
+    }
+}
+
 /* computeNonBonded_switch :336-424 */
 void orc_nonbonded_switch(orc_system *s)
 {
@@ -383,23 +417,14 @@ void orc_nonbonded_switch(orc_system *s)
     const double ctoff2 = s->ctoff * s->ctoff;
     const double cton2 = s->cton * s->cton;
     const double switch2 = 1. / (X3(ctoff2 - cton2));
-    for (int i = 0; i < s->natom; i++) {
-        double di[3], dj[3];
-        getc3(s, i, di);
-        const double qi = s->charge[i], epsi = s->eps[i], sigi = s->sig[i];
-        const int *row = s->neighList + (size_t)i * s->cap;
-        for (int k = 0; k < s->neighPair[i]; k++) {
-            int j = row[k];
-            getc3(s, j, dj);
-            double r2 = distance2(s, di, dj);
-            if (r2 <= ctoff2) {
-                double pe, pv;
-                switched_pair(s, r2, qi, s->charge[j], epsi, s->eps[j], sigi, s->sig[j],
-                              ctoff2, cton2, switch2, &pe, &pv);
-                lelec += pe;
-                levdw += pv;
-            }
-        }
+    const int nt = g_energy_threads;
+    if (nt > 1) {
+        #pragma omp parallel for schedule(dynamic, 16) num_threads(nt) reduction(+:lelec, levdw)
+        for (int i = 0; i < s->natom; i++)
+            switch_row(s, i, 0, s->neighPair[i], ctoff2, cton2, switch2, &lelec, &levdw);
+    } else {
+        for (int i = 0; i < s->natom; i++)
+            switch_row(s, i, 0, s->neighPair[i], ctoff2, cton2, switch2, &lelec, &levdw);
     }
     s->elec = CHGCHARMM * KCALTOIU * lelec;
     s->vdw = 4.0 * levdw;
@@ -615,29 +640,28 @@ double orc_moving_set_energy(orc_system *s, int first, int last)
     if (s->cutmode == ORC_CUT_SWITCH) {
         const double ctoff2 = s->ctoff * s->ctoff, cton2 = s->cton * s->cton;
         const double switch2 = 1. / (X3(ctoff2 - cton2));
-        for (int i = 0; i < N; i++) {
-            const int *row = s->neighList + (size_t)i * s->cap;
-            const int np = s->neighPair[i];
-            int k0 = 0, k1 = np;
-            if (!INSET(i)) {
-                if (i >= last) continue;
-                k0 = lower_bound_row(row, np, first);
-                k1 = lower_bound_row(row, np, last + 1);
-            }
-            double di[3], dj[3];
-            getc3(s, i, di);
-            for (int k = k0; k < k1; k++) {
-                int j = row[k];
-                getc3(s, j, dj);
-                double r2 = distance2(s, di, dj);
-                if (r2 <= ctoff2) {
-                    double pe, pv;
-                    switched_pair(s, r2, s->charge[i], s->charge[j], s->eps[i], s->eps[j],
-                                  s->sig[i], s->sig[j], ctoff2, cton2, switch2, &pe, &pv);
-                    lelec += pe; levdw += pv;
-                }
-            }
+        /* rows of moved atoms in full; of the others, the entries that are
+           moved atoms (rows hold ascending j > i) */
+#define MS_ROW(i_)                                                                 \
+        do {                                                                       \
+            const int *row = s->neighList + (size_t)(i_) * s->cap;                 \
+            const int np = s->neighPair[i_];                                       \
+            int k0 = 0, k1 = np;                                                   \
+            if (!INSET(i_)) {                                                      \
+                if ((i_) >= last) break;                                           \
+                k0 = lower_bound_row(row, np, first);                              \
+                k1 = lower_bound_row(row, np, last + 1);                           \
+            }                                                                      \
+            switch_row(s, (i_), k0, k1, ctoff2, cton2, switch2, &lelec, &levdw);   \
+        } while (0)
+        const int nt = g_energy_threads;
+        if (nt > 1) {
+            #pragma omp parallel for schedule(dynamic, 256) num_threads(nt) reduction(+:lelec, levdw)
+            for (int i = 0; i < N; i++) MS_ROW(i);
+        } else {
+            for (int i = 0; i < N; i++) MS_ROW(i);
         }
+#undef MS_ROW
     } else {
         for (int i = 0; i < N; i++) {
             for (int j = i + 1; j < N; j++) {

@@ -81,6 +81,8 @@ double orc_getE(orc_system *s, double ener[10]);
 void orc_nonbonded_full(orc_system *s);
 void orc_nonbonded14(orc_system *s);
 void orc_nonbonded_switch(orc_system *s);
+/* threads of the switched pair loops; 1 (default) = serial reference order */
+void orc_set_energy_threads(int n);
 void orc_nonbonded14_switch(orc_system *s);
 void orc_ebond(orc_system *s);
 void orc_eang(orc_system *s);

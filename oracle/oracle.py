@@ -15,6 +15,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libmc_oracle.so")
 REFLIB = os.path.join(HERE, "_ref", "libmcref.so")
+REFMCXLIB = os.path.join(HERE, "_ref", "libmcref_mcx.so")
 REF_TREE = "/root/reference"
 
 _dp = C.POINTER(C.c_double)
@@ -63,6 +64,8 @@ def build(ref: bool = True):
     subprocess.check_call(["make", "-s", "-C", HERE, "libmc_oracle.so"])
     if ref and os.path.isdir(REF_TREE):
         subprocess.check_call(["make", "-s", "-j8", "-C", HERE, "ref", "REF=" + REF_TREE])
+        if os.path.exists(os.path.join(HERE, "..", "mc_cpp_amd", "libmcx.so")):
+            subprocess.check_call(["make", "-s", "-C", HERE, "ref-mcx", "REF=" + REF_TREE])
 
 
 _lib = None
@@ -97,6 +100,7 @@ def lib():
         L.orc_rint.argtypes = [C.c_double]
         L.orc_rint.restype = C.c_double
         L.orc_nonbonded_switch_grad.argtypes = [P] + [_dp] * 6
+        L.orc_set_energy_threads.argtypes = [C.c_int]
         for f in ("orc_nonbonded_full", "orc_nonbonded_switch", "orc_nonbonded14", "orc_nonbonded14_switch",
                   "orc_ebond", "orc_eang", "orc_eub", "orc_edihe", "orc_eimpr"):
             getattr(L, f).argtypes = [P]
@@ -258,6 +262,12 @@ def rint(x):
 _ref = None
 
 
+def set_energy_threads(n):
+    """Threads of the oracle's switched pair loops (1 = the reference's
+    serial order, bit for bit; more = per-thread sums, for CPU timing)."""
+    lib().orc_set_energy_threads(int(n))
+
+
 def ref_available():
     return os.path.exists(REFLIB)
 
@@ -356,3 +366,86 @@ class Reference:
                 p = ln.split()
                 trials.append(int(p[1])); accepted.append(int(p[3]))
         return dict(dmax=out, trials=np.array(trials), accepted=np.array(accepted), text=text)
+
+
+_refmcx = None
+
+
+def refmcxlib():
+    global _refmcx
+    if _refmcx is None:
+        L = C.CDLL(REFMCXLIB)
+        L.refx_output.restype = C.c_char_p
+        L.refx_error.restype = C.c_char_p
+        L.refx_create.restype = C.c_void_p
+        L.refx_create.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_double, C.c_double, C.c_double,
+                                  C.c_int, C.c_double, C.c_char_p, C.c_double, C.c_double, C.c_double,
+                                  C.c_int, C.c_int]
+        L.refx_destroy.argtypes = [C.c_void_p]
+        L.refx_energy.argtypes = [C.c_void_p, _dp]
+        L.refx_get_coords.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.refx_mc.argtypes = [C.c_void_p, C.c_int, _ip, C.POINTER(C.c_char_p), _dp, _dp, _ip, C.c_int,
+                              C.c_uint64, _dp]
+        L.refx_mc.restype = C.c_longlong
+        L.refx_energies.argtypes = [C.c_void_p, _dp, C.c_longlong]
+        _refmcx = L
+    return _refmcx
+
+
+def refmcx_available():
+    return os.path.exists(REFMCXLIB)
+
+
+class ReferenceLoop(Reference):
+    """The reference's MC_metropolis::run over FField_MDBAS (gpu=False) or
+    over the drop-in FField_MCX of include/mcx_ffield_adapter.hpp
+    (gpu=True), through oracle/_ref/libmcref_mcx.so; every getE() the loop
+    makes is recorded."""
+
+    def __init__(self, cor, ff, sysm, gpu, device=0):
+        self.L = refmcxlib()
+        self.n = sysm.natom
+        cut = b"full" if sysm.cutmode == 0 else b"switch"
+        self.h = self.L.refx_create(cor.encode(), ff.encode(), self.PBC[sysm.pbtype], *map(float, sysm.box),
+                                    self.n, float(sysm.temp), cut, sysm.ctoff, sysm.cton, sysm.dcut,
+                                    int(bool(gpu)), int(device))
+        if not self.h:
+            raise RuntimeError("refx_create: " + self.L.refx_error().decode())
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.L.refx_destroy(self.h)
+        except Exception:
+            pass
+
+    def getE(self):
+        e = np.zeros(10)
+        self.L.refx_energy(self.h, _p(e, C.c_double))
+        return e
+
+    def coords(self):
+        x, y, z = np.zeros(self.n), np.zeros(self.n), np.zeros(self.n)
+        self.L.refx_get_coords(self.h, _p(x, C.c_double), _p(y, C.c_double), _p(z, C.c_double))
+        return x, y, z
+
+    def mc_run(self, moves, nsteps, seed):
+        nt = len(moves)
+        typ = np.array([m.type for m in moves], np.int32)
+        modes = (C.c_char_p * nt)(*[{1: b"residue", 2: b"all", 4: b"atom"}[m.mode] for m in moves])
+        dm = np.array([m.dmax for m in moves]); tg = np.array([m.target for m in moves])
+        ea = np.array([m.each for m in moves], np.int32)
+        out = np.zeros(nt)
+        ne = self.L.refx_mc(self.h, nt, _p(typ, C.c_int), modes, _p(dm, C.c_double), _p(tg, C.c_double),
+                            _p(ea, C.c_int), int(nsteps), C.c_uint64(seed), _p(out, C.c_double))
+        if ne < 0:
+            raise RuntimeError("MC_metropolis::run: " + self.L.refx_error().decode())
+        e = np.zeros(ne)
+        self.L.refx_energies(self.h, _p(e, C.c_double), ne)
+        text = self.L.refx_output().decode()
+        trials, accepted = [], []
+        for ln in text.splitlines():
+            if ln.startswith("TRIALS"):
+                p = ln.split()
+                trials.append(int(p[1])); accepted.append(int(p[3]))
+        return dict(dmax=out, trials=np.array(trials), accepted=np.array(accepted), energies=e, text=text)

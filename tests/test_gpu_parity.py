@@ -600,3 +600,29 @@ def test_incremental_list_equals_full_rebuild(case, monkeypatch):
     o.update_list()
     cnt_o, off_o, flat_o = o.list_csr()
     assert np.array_equal(cnt_o, l1[0]) and np.array_equal(flat_o, l1[1])
+
+
+def test_mc_replay_water12288_10k_against_fixture():
+    """The headline workload at its own size: 10,000 replayed Metropolis
+    moves on the 12,288-atom water box with the list updated every 50
+    moves, against the FP64 oracle's run of the same draws
+    (tests/golden/replay_water12288.npz, made by make_replay_water12288.py):
+    the same move picks and accept decisions at every step, dE within
+    1e-5 of max(|dE|, kT) (the bar of every replay here), the same final
+    dmax, total energy change and sampled final coordinates."""
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "replay_water12288.npz"))
+    s, _ = systems.water_box(seed=2)
+    ff = mcx.FField(s)
+    n = int(g["nsteps"])
+    res = ff.h.mc_run(s.moves, n, int(g["seed"]), rng_mode=mcx.MC_REPLAY, list_every=int(g["list_every"]))
+    ro = {"acc": g["acc"], "de": g["de"], "mvtyp": g["mvtyp"].astype(np.int32), "mvinst": g["mvinst"]}
+    kT = _kT(s)
+    n_ok = _compare_replay(res, ro, None, lambda de: 1e-5 * max(abs(de), kT))
+    assert n_ok == n, "accept sequence diverged at step %d" % n_ok
+    assert np.array_equal(res["mvtyp"], ro["mvtyp"]) and np.array_equal(res["mvinst"], ro["mvinst"])
+    assert np.allclose(res["dmax"], g["dmax"], rtol=1e-12, atol=0)
+    e0 = float(g["de_total"])
+    assert abs(float(res["de"][res["acc"] == 1].sum()) - e0) <= 1e-6 * abs(e0)
+    x, y, z = ff.get_coords()
+    p = g["pick"]
+    assert np.max(np.abs(np.stack([x[p], y[p], z[p]], 1) - g["xyz"])) < 1e-9

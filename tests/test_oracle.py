@@ -441,3 +441,38 @@ def test_gradient_vs_central_differences(oracle_lib):
         em = o.getE()[3] + o.getE()[4]
         s.x[i] = x0
         assert gx[i] == pytest.approx((ep - em) / (2 * h), rel=1e-5, abs=1e-3)
+
+
+def test_threaded_switch_loops_differ_by_rounding_only(oracle_lib):
+    """The all-cores variant of the switched loops (used for CPU timing)
+    sums the same pairs: within 1e-12 relative of the serial order, and
+    the serial order is restored bit for bit when set back to one thread."""
+    s, _ = systems.water_box(seed=3, nside=8, L=24.85)
+    o = O.Oracle(s)
+    o.update_list()
+    e1 = o.getE()
+    m1 = o.moving_set_energy(30, 32)
+    try:
+        O.set_energy_threads(4)
+        e4 = o.getE()
+        m4 = o.moving_set_energy(30, 32)
+    finally:
+        O.set_energy_threads(1)
+    assert abs(e4[0] - e1[0]) <= 1e-12 * abs(e1[0])
+    assert abs(m4 - m1) <= 1e-12 * abs(e1[0])
+    assert o.getE().tolist() == e1.tolist() and o.moving_set_energy(30, 32) == m1
+
+
+def test_full_size_replay_fixture_prefix(oracle_lib):
+    """The first 200 steps of tests/golden/replay_water12288.npz (the
+    10,000-move replay of the 12,288-atom box that the GPU loop is checked
+    against) are what the oracle computes now, bit for bit."""
+    g = np.load(os.path.join(HERE, "golden", "replay_water12288.npz"))
+    s, _ = systems.water_box(seed=2)
+    o = O.Oracle(s)
+    o.update_list(8)
+    res = o.mc_run(200, int(g["seed"]), mode=1, list_every=int(g["list_every"]), nthreads=8)
+    assert np.array_equal(res["mvtyp"], g["mvtyp"][:200].astype(np.int32))
+    assert np.array_equal(res["mvinst"], g["mvinst"][:200])
+    assert np.array_equal(res["acc"], g["acc"][:200])
+    assert res["de"].tolist() == g["de"][:200].tolist()
