@@ -10,10 +10,13 @@
 // bit for bit.
 //
 // Binning: atoms go into a periodic cell grid (cells at least
-// (ctoff+dcut)/reach wide) by one workgroup with LDS counters when the grid
-// has at most kBinCells cells (one launch), else by global counting kernels;
-// then each cell's atoms are put in ascending order, which makes the list
-// rows (and every sum over them) the same from build to build.
+// (ctoff+dcut)/reach wide): blocks of 1,024 atoms count into LDS histograms
+// added to global accumulators, the last block turns the totals into cell
+// and row-group starts and a table of row groups (grids of at most
+// kBinCells cells; larger ones count with global atomics over more
+// launches); the atoms are then placed, and each cell's atoms put in
+// ascending order, which makes the list rows (and every sum over them) the
+// same from build to build.
 //
 // Rows: a workgroup of four wavefronts takes a group of up to G atoms of
 // one cell and streams the surrounding cells' atoms once for all G rows
@@ -72,18 +75,18 @@ __device__ __forceinline__ void store_record(const DevSys &s, const CellGrid &g,
 
 // Block-wide exclusive scan of f(in[c]) for c in [0, n) into out (and into
 // out2 when given; out2 may alias `in`: chunk c is read before it is
-// written), 1024 threads; returns the total.  sm: 1024 ints of LDS.
-template <class F>
+// written), NT threads; returns the total.  sm: NT ints of LDS.
+template <int NT, class F>
 __device__ int block_scan(const int *in, int *out, int n, int *sm, int *carry, F f, int *out2 = nullptr)
 {
     if (threadIdx.x == 0) *carry = 0;
     __syncthreads();
-    for (int base = 0; base < n; base += 1024) {
+    for (int base = 0; base < n; base += NT) {
         const int c = base + threadIdx.x;
         const int v = c < n ? f(in[c]) : 0;
         sm[threadIdx.x] = v;
         __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {
+        for (int o = 1; o < NT; o <<= 1) {
             const int t = threadIdx.x >= o ? sm[threadIdx.x - o] : 0;
             __syncthreads();
             sm[threadIdx.x] += t;
@@ -94,7 +97,7 @@ __device__ int block_scan(const int *in, int *out, int n, int *sm, int *carry, F
             if (out2) out2[c] = *carry + sm[threadIdx.x] - v;
         }
         __syncthreads();
-        if (threadIdx.x == 1023) *carry += sm[1023];
+        if (threadIdx.x == NT - 1) *carry += sm[NT - 1];
         __syncthreads();
     }
     const int total = *carry;
@@ -110,34 +113,101 @@ struct Groups {
     __device__ int operator()(int v) const { return (v + g - 1) / g; }
 };
 
-// One-launch binning (ncell <= kBinCells): LDS counts, cell starts, row
-// groups (ceil(count / group) per cell), cell-ordered records; also clears
-// the overflow flag.
-__global__ __launch_bounds__(1024) void k_bin(DevSys s, CellGrid g, int group)
+// The row-group table: group k of cell c (ceil(count / group) per cell)
+// is (c, first record, atoms, 0), so that a row-group workgroup finds its
+// atoms with one load.  Threads of a block share out the cells.
+__device__ void write_groups(const CellGrid &g, int group, int nthreads)
 {
-    __shared__ int cnt[kBinCells];
-    __shared__ int sm[1024];
+    for (int c = threadIdx.x; c < g.ncell; c += nthreads) {
+        const int cnt = g.cell_count[c], gs = g.group_start[c], cs = g.cell_start[c];
+        for (int k = 0; k * group < cnt; k++)
+            g.group_info[gs + k] = make_int4(c, cs + k * group, min(group, cnt - k * group), 0);
+    }
+}
+
+// Binning (ncell <= kBinCells), pass 1: each block bins kBinAtoms atoms
+// (kBinAtoms / kBinT per thread, their loads in flight together) into an
+// LDS histogram and adds it to the global accumulators g.cell_acc, which
+// are zero between builds.  The last block to finish takes the totals
+// (resetting the accumulators), writes cell counts, cell starts, row-group
+// starts and the group table, clears the fill cursors and the overflow
+// flag, and re-arms the ticket.
+constexpr int kBinT = 256;
+constexpr int kBinPer = 4;
+constexpr int kBinAtoms = kBinT * kBinPer;
+
+__global__ __launch_bounds__(kBinT) void k_bin_count(DevSys s, CellGrid g, int group)
+{
+    __shared__ int hist[kBinCells];
+    __shared__ int sm[kBinT];
     __shared__ int carry;
+    __shared__ bool last;
     const int nc = g.ncell;
-    for (int c = threadIdx.x; c < nc; c += 1024) cnt[c] = 0;
-    if (threadIdx.x == 0) s.overflow[0] = 0;
+    for (int c = threadIdx.x; c < nc; c += kBinT) hist[c] = 0;
     __syncthreads();
-    for (int a = threadIdx.x; a < s.n; a += 1024) {
-        const int c = cell_of_atom(s, g, a);
-        g.cell_of[a] = c;
+    const int a0 = blockIdx.x * kBinAtoms + threadIdx.x;
+    int cell[kBinPer];
+#pragma unroll
+    for (int u = 0; u < kBinPer; u++) {
+        const int a = a0 + u * kBinT;
+        cell[u] = a < s.n ? cell_of_atom(s, g, a) : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < kBinPer; u++) {
+        const int a = a0 + u * kBinT;
+        if (cell[u] < 0) continue;
+        g.cell_of[a] = cell[u];
         g.ipos_ref[a] = s.ipos[a];
-        atomicAdd(&cnt[c], 1);
+        atomicAdd(&hist[cell[u]], 1);
     }
     __syncthreads();
-    for (int c = threadIdx.x; c < nc; c += 1024) g.cell_count[c] = cnt[c];
-    const int ng = block_scan(cnt, g.group_start, nc, sm, &carry, Groups{group});
+    for (int c = threadIdx.x; c < nc; c += kBinT)
+        if (hist[c]) atomicAdd(&g.cell_acc[c], hist[c]);
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(g.ticket, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    // the totals, read and reset by atomic exchange (device scope: every
+    // block's additions have landed before its ticket)
+    for (int c = threadIdx.x; c < nc; c += kBinT) {
+        const int v = atomicExch(&g.cell_acc[c], 0);
+        g.cell_count[c] = v;
+        hist[c] = v;
+        g.cursor[c] = 0;
+    }
+    if (threadIdx.x == 0) s.overflow[0] = 0;
+    __syncthreads();
+    const int ng = block_scan<kBinT>(hist, g.group_start, nc, sm, &carry, Groups{group});
     if (threadIdx.x == 0) g.group_start[nc] = ng;
-    // cell starts, also left in the LDS counters as the fill cursors
-    block_scan(cnt, g.cell_start, nc, sm, &carry, Identity{}, cnt);
-    // one cell: atom order; else unordered within cells (k_cell_sort)
-    for (int a = threadIdx.x; a < s.n; a += 1024) {
-        if (nc == 1) store_record(s, g, a, a);
-        else g.cell_tmp[atomicAdd(&cnt[g.cell_of[a]], 1)] = a;
+    block_scan<kBinT>(hist, g.cell_start, nc, sm, &carry, Identity{});
+    __syncthreads();
+    write_groups(g, group, kBinT);
+    if (threadIdx.x == 0) *g.ticket = 0u;
+}
+
+// Pass 2: atoms into their cells' ranges (unordered; k_cell_sort orders
+// them).  One cell: the records in atom order, no sort.
+__global__ __launch_bounds__(kBinT) void k_bin_fill(DevSys s, CellGrid g)
+{
+    const int a0 = blockIdx.x * kBinAtoms + threadIdx.x;
+    if (g.ncell == 1) {
+        for (int u = 0; u < kBinPer; u++)
+            if (a0 + u * kBinT < s.n) store_record(s, g, a0 + u * kBinT, a0 + u * kBinT);
+        return;
+    }
+    int cell[kBinPer];
+#pragma unroll
+    for (int u = 0; u < kBinPer; u++) {
+        const int a = a0 + u * kBinT;
+        cell[u] = a < s.n ? g.cell_of[a] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < kBinPer; u++) {
+        if (cell[u] < 0) continue;
+        const int c = cell[u];
+        g.cell_tmp[g.cell_start[c] + atomicAdd(&g.cursor[c], 1)] = a0 + u * kBinT;
     }
 }
 
@@ -156,9 +226,11 @@ __global__ __launch_bounds__(1024) void k_cell_scan(CellGrid g, int group)
 {
     __shared__ int sm[1024];
     __shared__ int carry;
-    block_scan(g.cell_count, g.cell_start, g.ncell, sm, &carry, Identity{});
-    const int ng = block_scan(g.cell_count, g.group_start, g.ncell, sm, &carry, Groups{group});
+    block_scan<1024>(g.cell_count, g.cell_start, g.ncell, sm, &carry, Identity{});
+    const int ng = block_scan<1024>(g.cell_count, g.group_start, g.ncell, sm, &carry, Groups{group});
     if (threadIdx.x == 0) g.group_start[g.ncell] = ng;
+    __syncthreads();
+    write_groups(g, group, 1024);
 }
 
 __global__ void k_cell_fill(DevSys s, CellGrid g)
@@ -297,26 +369,15 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_build_groups(DevSys s, CellG
     constexpr int NT = 64 * kRowWaves;
     extern __shared__ uint32_t bitmap[];     // G rows of `words` words
     __shared__ int run_start[kMaxRuns], run_len[kMaxRuns], run_pref[kMaxRuns + 1];
-    __shared__ int s_cell, s_p0, s_na;
     __shared__ int s_a[G];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int grp = blockIdx.x;
-    if (grp >= g.group_start[g.ncell]) return;       // uniform over the block
-    if (tid == 0) {
-        int lo = 0, hi = g.ncell - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (g.group_start[mid] <= grp) lo = mid; else hi = mid - 1;
-        }
-        const int k = grp - g.group_start[lo];
-        s_cell = lo;
-        s_p0 = g.cell_start[lo] + k * G;
-        s_na = min(G, g.cell_count[lo] - k * G);
-    }
+    const int ngrp = g.group_start[g.ncell];
+    const int4 gi = g.group_info[grp];                // (cell, first record, atoms); valid when grp < ngrp
+    if (grp >= ngrp) return;                          // uniform over the block
     for (int k = tid; k < G * words; k += NT) bitmap[k] = 0u;
-    __syncthreads();
-    const int na = s_na, ci = s_cell;
-    if (tid < na) s_a[tid] = g.cell_atoms[s_p0 + tid];
+    const int na = gi.z, ci = gi.x;
+    if (tid < na) s_a[tid] = g.cell_atoms[gi.y + tid];
     const int nrun = 2 * cell_runs(g, ci, tid, run_start, run_len);
     __syncthreads();
     // the runs as one index space: prefix of their lengths (wavefront 0)
@@ -343,13 +404,16 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_build_groups(DevSys s, CellG
     unsigned fmask = 0u;
 #pragma unroll
     for (int q = 0; q < G; q++) {
-        const int aa = q < na ? s_a[q] : s_a[0];
-        if (q < na && s.frozen[aa]) fmask |= 1u << q;
         if (PBC) {
-            pi[q] = s.ipos[aa];
+            // the row atoms' own cell records: fixed-point position, 2*atom + frozen
+            pi[q] = g.cfix[gi.y + (q < na ? q : 0)];
+            if (q < na && (pi[q].w & 1)) fmask |= 1u << q;
             ox[q] = q < na ? (float)wdiff(pi[q].x, pc.x) * ux : 1e18f;
             oy[q] = (float)wdiff(pi[q].y, pc.y) * uy;
             oz[q] = (float)wdiff(pi[q].z, pc.z) * uz;
+        } else {
+            const int aa = q < na ? s_a[q] : s_a[0];
+            if (q < na && s.frozen[aa]) fmask |= 1u << q;
         }
     }
     const unsigned live = na >= 32 ? ~0u : ((1u << na) - 1u);
@@ -463,7 +527,9 @@ void launch_list_build(const DevSys &s, const CellGrid &g, hipStream_t st)
     int G = 8;
     while (G > 1 && per_row * G > 60 * 1024) G >>= 1;
     if (g.ncell <= kBinCells) {
-        hipLaunchKernelGGL(k_bin, dim3(1), dim3(1024), 0, st, s, g, G);
+        const int nb = (s.n + kBinAtoms - 1) / kBinAtoms;
+        hipLaunchKernelGGL(k_bin_count, dim3(nb), dim3(kBinT), 0, st, s, g, G);
+        hipLaunchKernelGGL(k_bin_fill, dim3(nb), dim3(kBinT), 0, st, s, g);
     } else {
         const int nb = (s.n + 255) / 256;
         hipLaunchKernelGGL(k_zero_int, dim3(64), dim3(256), 0, st, g.cell_count, g.ncell);

@@ -23,8 +23,12 @@
 
 namespace {
 
+// 512 threads: eight wavefronts of up to 256 VGPRs each, which the
+// speculative kernel's proposal, walk and resolve state needs without
+// spilling to scratch (at 1,024 threads the 128-VGPR limit spilled ~600
+// bytes per lane, and every round's release fence wrote the spills back)
 #ifndef MCX_MC_THREADS
-#define MCX_MC_THREADS 1024
+#define MCX_MC_THREADS 512
 #endif
 #ifndef MCX_MC_UNROLL
 #define MCX_MC_UNROLL 4

@@ -222,6 +222,16 @@ void setup_grid(mcx_ctx *h)
     h->ipos_ref.alloc(h->n);
     h->cell_tmp.alloc(h->n);
     h->group_start.alloc(g.ncell + 1);
+    h->group_info.alloc((size_t)h->n + g.ncell + 1);
+    h->cell_acc.alloc(g.ncell);
+    h->bin_ticket.alloc(1);
+    // the binning's accumulators and ticket are zero between builds (its
+    // last block resets them); set here, stream-ordered after earlier builds
+    HIPCHK(hipMemsetAsync(h->cell_acc.p, 0, sizeof(int) * g.ncell, h->stream));
+    HIPCHK(hipMemsetAsync(h->bin_ticket.p, 0, sizeof(unsigned), h->stream));
+    g.group_info = h->group_info.p;
+    g.cell_acc = h->cell_acc.p;
+    g.ticket = h->bin_ticket.p;
     g.cpos = h->cpos.p;
     g.cfix = h->cfix.p;
     g.ipos_ref = h->ipos_ref.p;

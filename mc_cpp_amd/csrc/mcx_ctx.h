@@ -77,7 +77,9 @@ struct mcx_ctx {
     DBuf<double4> cpos;
     DBuf<int> nbr, cnt, half, overflow;
     int cap = 0;
-    DBuf<int> cell_of, cell_start, cell_count, cell_atoms, cursor, group_start, cell_tmp;
+    DBuf<int> cell_of, cell_start, cell_count, cell_atoms, cursor, group_start, cell_tmp, cell_acc;
+    DBuf<int4> group_info;
+    DBuf<unsigned> bin_ticket;
     DBuf<int4> cfix, ipos_ref;
     CellGrid grid{};
     DBuf<int2> tiles;
@@ -184,9 +186,12 @@ struct mcx_ctx {
         adj_type.release();
         mvb.release();
         DBuf<int> *ib[] = {&frozen_d, &excl_off_d, &excl_idx_d, &xfull_off_d, &xfull_idx_d, &nbr, &cnt, &half, &overflow, &cell_of,
-                           &cell_start, &cell_count, &cell_atoms, &cursor, &group_start, &cell_tmp, &adj_off, &adj_kk};
+                           &cell_start, &cell_count, &cell_atoms, &cursor, &group_start, &cell_tmp, &adj_off, &adj_kk,
+                           &cell_acc};
         for (auto *b : ib) b->release();
         tiles.release();
+        group_info.release();
+        bin_ticket.release();
         DBuf<int> *pb[] = {&p_moved_flag, &p_moved_list, &p_moved_cnt, &p_stale_flag, &p_stale_list, &p_stale_cnt,
                            &p_edit_cnt, &p_edit_val, &p_touched, &p_touched_cnt, &p_newrow, &p_newcnt, &p_newhalf};
         for (auto *b : pb) b->release();

@@ -168,6 +168,9 @@ struct CellGrid {
     int4 *ipos_ref;          // fixed-point positions at this build (displacement test)
     int *cell_tmp;           // binned atoms before the in-cell sort
     int *group_start;        // [ncell + 1] first row group of each cell (periodic build)
+    int4 *group_info;        // [n + ncell] row group -> (cell, first record, atoms, 0)
+    int *cell_acc;           // [ncell] binning accumulators, zero between builds
+    unsigned *ticket;        // [1] blocks of the binning pass done, zero between builds
 };
 void launch_list_build(const DevSys &s, const CellGrid &g, hipStream_t st);
 size_t list_build_lds_bytes(int n);
@@ -240,7 +243,7 @@ void launch_mc_chunk(const DevSys &s, const DevTerms &t, const DevMoves &mv, con
 // round evaluates the proposal g moves ahead on the assumption that the
 // moves before it are rejected; the rounds resolve in chain order.
 constexpr int kMaxWide = 32;        // workgroups of one launch
-constexpr int kDefaultWide = 8;     // proposals per round unless MCX_WIDE says otherwise
+constexpr int kDefaultWide = 12;    // proposals per round unless MCX_WIDE says otherwise
 constexpr int kDefaultSplit = 2;    // workgroups per proposal unless MCX_WIDE_SPLIT says otherwise
 constexpr int kWideRing = 512;      // draws generated ahead (power of two)
 // the draw state of a wide run: the common layout (kRngWords words, last
