@@ -19,10 +19,12 @@ Large mixed system (same run): the ~100k-atom water + chain box -- pair
 kernels, list build, and the device loop with the 50-move list schedule
 and with displacement-triggered rebuilds -> "large_mixed".
 
-Force-bias (same run): residue force-bias moves (A = 0.01 A^2) against
-plain residue translations on the 12,288-atom box, moves/s and acceptance;
-the adjoint kernel's time on that box and a 64-configuration batched
-adjoint call -> "force_bias".
+Force-bias (same run): residue force-bias moves (A = 0.01 A^2, drift with
+F in kcal/mol/A; and the drift at the Metropolis test's own beta, A adapted)
+against plain residue translations on the 12,288-atom box: moves/s,
+acceptance and accepted mean-square displacement per second; the adjoint
+kernel's time on that box and a 64-configuration batched adjoint call ->
+"force_bias".
 
 Also measured in the same run (HIP events on the library's own stream):
   * "roofline": the Metropolis kernel of the headline loop -- algorithmic
@@ -115,14 +117,25 @@ def force_bias_vs_translation(args):
     from mc_cpp_amd import mcx, systems
     out = {}
     for name, mt in (("force_bias", systems.MoveType(systems.MV_FBIAS, systems.MODE_RESIDUE, 0.01, 50.0, 0)),
+                     # drift at the test's own beta; A adapted toward 50 % acceptance every 100 moves
+                     ("force_bias_test_beta", systems.MoveType(systems.MV_FBIAS_ACC, systems.MODE_RESIDUE, 5e-6,
+                                                               50.0, 100)),
                      ("translation", systems.MoveType(systems.MV_TRN, systems.MODE_RESIDUE, 0.3, 50.0, 0))):
         s, _ = systems.water_box(seed=2)
         ff = mcx.FField(s)
         ff.h.mc_run([mt], 2000, 5, rng_mode=mcx.MC_PHILOX, list_every=50, record=False)   # warm-up
-        r = ff.h.mc_run([mt], args.fb_moves, 6, rng_mode=mcx.MC_PHILOX, list_every=50, record=False)
+        r = ff.h.mc_run([mt], args.fb_moves, 6, rng_mode=mcx.MC_PHILOX, list_every=50, record=True)
+        acc = r["acc"].astype(bool)
+        sq = float((r["disp"][acc] ** 2).sum())          # A^2, accepted molecule displacements
         out[name] = {"moves_per_s": args.fb_moves / r["seconds"],
-                     "acceptance": float(r["accepted"][0]) / args.fb_moves, "moves": args.fb_moves}
+                     "acceptance": float(r["accepted"][0]) / args.fb_moves, "moves": args.fb_moves,
+                     "final_step": float(r["dmax"][0]),
+                     "accepted_sq_disp_A2_per_s": sq / r["seconds"],
+                     "mean_sq_disp_per_move_A2": sq / args.fb_moves}
     out["force_bias_over_translation"] = out["force_bias"]["moves_per_s"] / out["translation"]["moves_per_s"]
+    for name in ("force_bias", "force_bias_test_beta"):
+        out[name]["sq_disp_per_s_over_translation"] = (out[name]["accepted_sq_disp_A2_per_s"] /
+                                                        out["translation"]["accepted_sq_disp_A2_per_s"])
     # adjoint kernel: one launch over the 12,288-atom box (HIP events)
     s, _ = systems.water_box(seed=2)
     h = mcx.FField(s).h

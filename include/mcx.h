@@ -50,6 +50,7 @@ extern "C" {
 #define MCX_MV_TRN 1
 #define MCX_MV_ROT 2
 #define MCX_MV_FBIAS 3   /* force-bias (smart MC) translation; not in the reference */
+#define MCX_MV_FBIAS_ACC 4 /* the same with the drift taken at the Metropolis test's own beta; not in the reference */
 #define MCX_MODE_RESIDUE 1
 #define MCX_MODE_ALL     2
 #define MCX_MODE_ATOM    4
@@ -151,7 +152,7 @@ int mcx_delta_energy(mcx_ctx *h, int first, int last, const double *nx, const do
 /* One Monte Carlo move type, as List_Moves::addNewMoveType registers it
  * (src/List_Moves.cpp:170-211,257-457).  sele: 0/1 per atom or NULL = all. */
 typedef struct mcx_move_type {
-    int type;          /* MCX_MV_TRN / MCX_MV_ROT / MCX_MV_FBIAS */
+    int type;          /* MCX_MV_TRN / MCX_MV_ROT / MCX_MV_FBIAS / MCX_MV_FBIAS_ACC */
     int mode;          /* MCX_MODE_RESIDUE / MCX_MODE_ATOM / MCX_MODE_ALL */
     double dmax;       /* A for TRN, degrees for ROT, the step A (A^2) for FBIAS */
     double target;     /* acceptance target in % */
@@ -226,7 +227,11 @@ typedef struct mcx_mc_trace {
  * formed as for TRN/ROT moves (dE in iu, as the reference's test does), ln q
  * ratio = (|dx - beta A F_old|^2 - |-dx - beta A F_new|^2) / (4A).  Draws per move: move type, instance, four uniforms for three
  * normals (Box-Muller on 1 - u), and the acceptance uniform only when the
- * exponent is negative.  Batches (mcx_batch_create) refuse FBIAS types. */
+ * exponent is negative.  MCX_MV_FBIAS_ACC moves are the same with F in
+ * iu/A (418.4 times the kcal/mol/A force): the drift then belongs to the
+ * density the test samples, exp(-beta E) with E in iu, 418.4 times colder
+ * than 1/(RT) in kcal/mol, so A must be ~418 times smaller for the same
+ * drift.  Batches (mcx_batch_create) refuse force-bias types. */
 int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int nsteps, uint64_t seed,
                       int rng_mode, int list_every, mcx_mc_stats *stats, const mcx_mc_trace *trace);
 
@@ -235,9 +240,13 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
  * replays that replica's proposals.  Not in the reference. */
 int mcx_set_rng_stream(mcx_ctx *h, uint32_t stream);
 
-/* Adjoint of the switched nonbonded energy (list-based): dE/dx,dy,dz per
- * atom and dE/deps, dE/dsig, dE/dq per atom (iu, with eps/sig in the
- * AtomList convention).  Not in the reference. */
+/* Adjoint of the nonbonded energy elec + vdw: SWITCH mode the list pairs
+ * inside the cutoff (computeNonBonded_switch, src/FField_MDBAS.cpp:336-424),
+ * FULL mode every non-excluded pair (computeNonBonded_full, :194-265), and in
+ * both the 1-4 pairs (:267-318, :426-488).  dE/dx,dy,dz per atom and dE/deps,
+ * dE/dsig, dE/dq per atom (iu, with eps/sig in the AtomList convention; the
+ * 1-4 pairs' own eps14/sig14 derivatives come from
+ * mcx_nonbonded_grad_device).  Not in the reference. */
 int mcx_nonbonded_grad(mcx_ctx *h, double *gx, double *gy, double *gz,
                        double *geps, double *gsig, double *gq);
 
@@ -247,8 +256,8 @@ int mcx_nonbonded_grad(mcx_ctx *h, double *gx, double *gy, double *gz,
 int mcx_nonbonded_grad_types(mcx_ctx *h, int ntypes, const int *type, double *gx, double *gy, double *gz,
                              double *geps, double *gsig, double *gq);
 
-/* Batched adjoint: the gradients of nsys systems (SWITCH mode, one device)
- * in one launch.  g6[k] receives system k's 6*n values: gx, gy, gz, geps,
+/* Batched adjoint: the gradients of nsys systems (SWITCH mode, 1-4 pairs
+ * included, one device) in one launch.  g6[k] receives system k's 6*n values: gx, gy, gz, geps,
  * gsig, gq back to back.  Errors are reported on hs[0].  Not in the
  * reference. */
 int mcx_nonbonded_grad_batch(mcx_ctx *const *hs, int nsys, double *const *g6);
@@ -256,6 +265,21 @@ int mcx_nonbonded_grad_batch(mcx_ctx *const *hs, int nsys, double *const *g6);
 /* Device pointer access for zero-copy interop (PyTorch tensors, HIP code):
  * which = 0 x, 1 y, 2 z (double[N] master coordinates). */
 int mcx_device_coords(mcx_ctx *h, int which, void **ptr);
+
+/* The caller wrote the master coordinates in place on the device (through
+ * mcx_device_coords): the fixed-point mirror is recomputed on the device,
+ * the pair list marked stale and the host copy refreshed when next needed.
+ * Work the caller queued on its own streams must have completed.  Not in
+ * the reference. */
+int mcx_coords_updated(mcx_ctx *h);
+
+/* The adjoint of mcx_nonbonded_grad with its outputs in device memory (no
+ * host round trip): gpos[3 n] = dE/dx, dE/dy, dE/dz per atom; when ntypes
+ * > 0, gtype[5 ntypes] = dE/deps, dE/dsig, dE/dq, dE/deps14, dE/dsig14
+ * summed per atom type in FP64 in a fixed order (type: n host indices in
+ * [0, ntypes); pass type[i] = i, ntypes = n for per-atom values).  Runs on
+ * the handle's stream and returns after it.  Not in the reference. */
+int mcx_nonbonded_grad_device(mcx_ctx *h, double *gpos, int ntypes, const int *type, double *gtype);
 
 /* Timing probe used by the benchmark: runs `reps` back-to-back launches of
  * kernel `which` (0 full O(N^2) energy, 1 switched list energy, 2 list

@@ -565,7 +565,7 @@ __global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s0, DevTerms t, DevMov
                 // 418.4), A = dmax (A^2) and beta = 1/(R T) in mol/kcal
                 force_old(s, m);
                 if (tid == 0) {
-                    const double A = dmax[m.acc], bA = arg.beta * A / MCX_KCALTOIU, sq = sqrt(2.0 * A);
+                    const double A = dmax[m.acc], bA = arg.beta * A * mv.fb_scale[m.acc], sq = sqrt(2.0 * A);
                     for (int c = 0; c < 3; c++) {
                         m.xi[c] = m.r[c];
                         m.r[c] = bA * m.fold[c] + sq * m.xi[c];
@@ -594,7 +594,7 @@ __global__ __launch_bounds__(kMCT) void k_mc_chunk(DevSys s0, DevTerms t, DevMov
                 // as the reference's test forms it (dE in iu), so force-bias
                 // and TRN/ROT moves leave the same density invariant
                 // whatever the drift's units
-                const double A = dmax[tp], bA = arg.beta * A / MCX_KCALTOIU;
+                const double A = dmax[tp], bA = arg.beta * A * mv.fb_scale[tp];
                 double fw = 0.0, rv = 0.0;
                 for (int c = 0; c < 3; c++) {
                     const double a = m.r[c] - bA * m.fold[c], b = -m.r[c] - bA * m.fnew[c];
@@ -1419,6 +1419,16 @@ void launch_delta_energy(const DevSys &s, const DevTerms &t, const DevMoves &mv,
     hipLaunchKernelGGL(k_delta, dim3(1), dim3(kMCT), 0, st, s, t, mv, first, last, newxyz, de);
 }
 
+// The fixed-point mirror of every atom from the FP64 master coordinates
+// (after a caller wrote them in place), charge bits kept.
+__global__ void k_mirror(DevSys s)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
+        const int qb = s.ipos[i].w;
+        s.ipos[i] = mcx_fixed4(s, s.x[i], s.y[i], s.z[i], __int_as_float(qb));
+    }
+}
+
 hipError_t launch_mc_wide(const DevSys &s, const DevTerms &t, const DevMoves &mv, const MCRunArgs &a,
                           const WideArgs &w, hipStream_t st)
 {
@@ -1465,4 +1475,9 @@ void launch_mc_chunk(const DevSys &s, const DevTerms &t, const DevMoves &mv, con
         hipLaunchKernelGGL((k_mc_chunk<MCX_PBC_CUBIC, MCX_CUT_FULL>), dim3(1), dim3(kMCT), 0, st, s, t, mv, a);
     else
         hipLaunchKernelGGL((k_mc_chunk<-1, -1>), dim3(1), dim3(kMCT), 0, st, s, t, mv, a);
+}
+
+void launch_mirror_from_coords(const DevSys &s, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mirror, dim3((s.n + 255) / 256), dim3(256), 0, st, s);
 }

@@ -197,6 +197,23 @@ void upload_terms(mcx_ctx *h)
     h->xfull_idx_d.upload(xi.data(), xi.size(), h->stream);
     h->xfull_off_h = xo;
     h->xfull_idx_h = xi;
+    // both-direction 1-4 partners, one entry per 1-4 pair record (the
+    // adjoint kernel's per-atom rows)
+    std::vector<int> po(N + 1, 0), pi;
+    const int n14 = (int)h->pairs14.size() / 2;
+    for (int k = 0; k < n14; k++) { po[h->pairs14[2 * k] + 1]++; po[h->pairs14[2 * k + 1] + 1]++; }
+    for (int a = 0; a < N; a++) po[a + 1] += po[a];
+    pi.resize(po[N]);
+    {
+        std::vector<int> cur(po.begin(), po.end() - 1);
+        for (int k = 0; k < n14; k++) {
+            const int a = h->pairs14[2 * k], b = h->pairs14[2 * k + 1];
+            pi[cur[a]++] = b;
+            pi[cur[b]++] = a;
+        }
+    }
+    h->p14_off_d.upload(po.data(), po.size(), h->stream);
+    h->p14_idx_d.upload(pi.data(), pi.size(), h->stream);
 }
 
 void setup_grid(mcx_ctx *h)
@@ -394,6 +411,7 @@ void upload_mirror(mcx_ctx *h)
 {
     const int n = h->n;
     if (!h->have_coords) return;
+    if (h->host_stale) sync_host_coords(h);
     std::vector<int4> m(n);
     const bool pbc = h->pbtype != MCX_PBC_NONE;
     for (int i = 0; i < n; i++) {
@@ -433,6 +451,7 @@ void upload_coords(mcx_ctx *h, const double *x, const double *y, const double *z
 void sync_host_coords(mcx_ctx *h)
 {
     const int n = h->n;
+    h->host_stale = false;
     h->hx.resize(n); h->hy.resize(n); h->hz.resize(n);
     HIPCHK(hipMemcpyAsync(h->hx.data(), h->x.p, n * 8, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipMemcpyAsync(h->hy.data(), h->y.p, n * 8, hipMemcpyDeviceToHost, h->stream));
@@ -481,11 +500,13 @@ DevMoves build_moves(mcx_ctx *h, const mcx_move_type *types, int ntypes, MoveBuf
     const std::vector<int> resLast = residue_last_table(h);
     for (int t = 0; t < ntypes; t++) {
         const mcx_move_type &m = types[t];
-        if (m.type != MCX_MV_TRN && m.type != MCX_MV_ROT && m.type != MCX_MV_FBIAS)
-            throw mcx_error(MCX_ERR_ARG, "move type must be TRN, ROT or FBIAS");
+        if (m.type != MCX_MV_TRN && m.type != MCX_MV_ROT && m.type != MCX_MV_FBIAS && m.type != MCX_MV_FBIAS_ACC)
+            throw mcx_error(MCX_ERR_ARG, "move type must be TRN, ROT, FBIAS or FBIAS_ACC");
         if (m.mode == MCX_MODE_ATOM && m.type == MCX_MV_ROT) throw mcx_error(MCX_ERR_ARG, "rotation of a single atom");
-        if (m.type == MCX_MV_FBIAS && !(m.dmax > 0.0)) throw mcx_error(MCX_ERR_ARG, "force-bias step A must be > 0");
-        mv.type[t] = m.type;
+        const bool fbias = m.type == MCX_MV_FBIAS || m.type == MCX_MV_FBIAS_ACC;
+        if (fbias && !(m.dmax > 0.0)) throw mcx_error(MCX_ERR_ARG, "force-bias step A must be > 0");
+        mv.type[t] = fbias ? MCX_MV_FBIAS : m.type;
+        mv.fb_scale[t] = m.type == MCX_MV_FBIAS_ACC ? 1.0 : 1.0 / MCX_KCALTOIU;
         mv.dmax0[t] = m.dmax;
         mv.target[t] = m.target;
         mv.each[t] = m.each;
@@ -1009,7 +1030,7 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
             a.record_moves = 1;
         }
         for (int k = 0; k < ntypes; k++)
-            if (types[k].type == MCX_MV_FBIAS) a.spec = 0;
+            if (types[k].type == MCX_MV_FBIAS || types[k].type == MCX_MV_FBIAS_ACC) a.spec = 0;
         // speculative rounds over several workgroups (k_mc_wide) for the
         // same move sets; MCX_WIDE=<n> sets the workgroup count, 0 or 1
         // keeps the single-workgroup read-ahead kernel
@@ -1280,20 +1301,42 @@ int mcx_set_rng_stream(mcx_ctx *h, uint32_t stream)
     return guarded(h, [&] { h->rng_stream = stream; });
 }
 
+namespace {
+
+// The adjoint of elec + vdw into the device buffer g8 (8 n values), on the
+// handle's stream: SWITCH mode over the list (built if stale), FULL mode
+// over all pairs; 1-4 pairs in both.
+void grad_device(mcx_ctx *h, double *g8)
+{
+    if (!h->have_excl || !h->have_coords || !h->have_atoms) throw mcx_error(MCX_ERR_STATE, "system not set up");
+    if (h->cutmode != MCX_CUT_FULL && !h->list_valid) build_list(h);
+    launch_nonbonded_grad(h->sys(), g8, h->stream);
+    HIPCHK(hipGetLastError());
+}
+
+void check_types(const mcx_ctx *h, int ntypes, const int *type)
+{
+    if (ntypes < 1 || !type) throw mcx_error(MCX_ERR_ARG, "atom types required");
+    for (int i = 0; i < h->n; i++)
+        if (type[i] < 0 || type[i] >= ntypes) throw mcx_error(MCX_ERR_ARG, "atom type out of range");
+}
+
+}  // namespace
+
 int mcx_nonbonded_grad(mcx_ctx *h, double *gx, double *gy, double *gz, double *geps, double *gsig, double *gq)
 {
     return guarded(h, [&] {
-        if (h->cutmode != MCX_CUT_SWITCH) throw mcx_error(MCX_ERR_UNSUPPORTED, "gradients need SWITCH mode");
-        if (!h->list_valid) build_list(h);
         const int n = h->n;
         DBuf<double> g;
-        g.alloc((size_t)6 * n);
-        launch_nonbonded_grad(h->sys(), g.p, h->stream);
-        HIPCHK(hipGetLastError());
+        struct Release {
+            DBuf<double> &a;
+            ~Release() { a.release(); }
+        } rel{g};
+        g.alloc((size_t)8 * n);
+        grad_device(h, g.p);
         std::vector<double> hg((size_t)6 * n);
         HIPCHK(hipMemcpyAsync(hg.data(), g.p, hg.size() * 8, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
-        g.release();
         double *outs[6] = {gx, gy, gz, geps, gsig, gq};
         for (int c = 0; c < 6; c++)
             if (outs[c]) std::copy(hg.begin() + (size_t)c * n, hg.begin() + (size_t)(c + 1) * n, outs[c]);
@@ -1304,12 +1347,8 @@ int mcx_nonbonded_grad_types(mcx_ctx *h, int ntypes, const int *type, double *gx
                              double *geps, double *gsig, double *gq)
 {
     return guarded(h, [&] {
-        if (h->cutmode != MCX_CUT_SWITCH) throw mcx_error(MCX_ERR_UNSUPPORTED, "gradients need SWITCH mode");
-        if (ntypes < 1 || !type) throw mcx_error(MCX_ERR_ARG, "atom types required");
+        check_types(h, ntypes, type);
         const int n = h->n;
-        for (int i = 0; i < n; i++)
-            if (type[i] < 0 || type[i] >= ntypes) throw mcx_error(MCX_ERR_ARG, "atom type out of range");
-        if (!h->list_valid) build_list(h);
         DBuf<double> g, out;
         DBuf<int> ty;
         struct Release {
@@ -1317,13 +1356,13 @@ int mcx_nonbonded_grad_types(mcx_ctx *h, int ntypes, const int *type, double *gx
             DBuf<int> &c;
             ~Release() { a.release(); b.release(); c.release(); }
         } rel{g, out, ty};
-        g.alloc((size_t)6 * n);
-        out.alloc((size_t)3 * ntypes);
+        g.alloc((size_t)8 * n);
+        out.alloc((size_t)5 * ntypes);
         ty.upload(type, n, h->stream);
-        launch_nonbonded_grad(h->sys(), g.p, h->stream);
+        grad_device(h, g.p);
         launch_type_sums(g.p, n, ty.p, ntypes, out.p, h->stream);
         HIPCHK(hipGetLastError());
-        std::vector<double> hg((size_t)3 * n), ho((size_t)3 * ntypes);
+        std::vector<double> hg((size_t)3 * n), ho((size_t)5 * ntypes);
         HIPCHK(hipMemcpyAsync(hg.data(), g.p, hg.size() * 8, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipMemcpyAsync(ho.data(), out.p, ho.size() * 8, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
@@ -1331,10 +1370,51 @@ int mcx_nonbonded_grad_types(mcx_ctx *h, int ntypes, const int *type, double *gx
         for (int c = 0; c < 3; c++)
             if (pos[c]) std::copy(hg.begin() + (size_t)c * n, hg.begin() + (size_t)(c + 1) * n, pos[c]);
         for (int t = 0; t < ntypes; t++) {
-            if (geps) geps[t] = ho[3 * t];
-            if (gsig) gsig[t] = ho[3 * t + 1];
-            if (gq) gq[t] = ho[3 * t + 2];
+            if (geps) geps[t] = ho[5 * t];
+            if (gsig) gsig[t] = ho[5 * t + 1];
+            if (gq) gq[t] = ho[5 * t + 2];
         }
+    });
+}
+
+int mcx_coords_updated(mcx_ctx *h)
+{
+    return guarded(h, [&] {
+        if (!h->have_coords) throw mcx_error(MCX_ERR_STATE, "no coordinates");
+        launch_mirror_from_coords(h->sys(), h->stream);
+        HIPCHK(hipGetLastError());
+        h->list_valid = false;
+        h->patch_valid = false;
+        h->host_stale = true;
+        HIPCHK(hipStreamSynchronize(h->stream));
+    });
+}
+
+int mcx_nonbonded_grad_device(mcx_ctx *h, double *gpos, int ntypes, const int *type, double *gtype)
+{
+    return guarded(h, [&] {
+        if (!gpos) throw mcx_error(MCX_ERR_ARG, "null gradient buffer");
+        if (ntypes > 0) {
+            check_types(h, ntypes, type);
+            if (!gtype) throw mcx_error(MCX_ERR_ARG, "null per-type gradient buffer");
+        }
+        const int n = h->n;
+        DBuf<double> g;
+        DBuf<int> ty;
+        struct Release {
+            DBuf<double> &a;
+            DBuf<int> &b;
+            ~Release() { a.release(); b.release(); }
+        } rel{g, ty};
+        g.alloc((size_t)8 * n);
+        grad_device(h, g.p);
+        HIPCHK(hipMemcpyAsync(gpos, g.p, (size_t)3 * n * 8, hipMemcpyDeviceToDevice, h->stream));
+        if (ntypes > 0) {
+            ty.upload(type, n, h->stream);
+            launch_type_sums(g.p, n, ty.p, ntypes, gtype, h->stream);
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipStreamSynchronize(h->stream));
     });
 }
 
@@ -1360,11 +1440,12 @@ int mcx_nonbonded_grad_batch(mcx_ctx *const *hs, int nsys, double *const *g6)
         for (int k = 0; k < nsys; k++) {
             mcx_ctx *h = hs[k];
             if (h->device != h0->device) throw mcx_error(MCX_ERR_ARG, "systems on different devices");
-            if (h->cutmode != MCX_CUT_SWITCH) throw mcx_error(MCX_ERR_UNSUPPORTED, "gradients need SWITCH mode");
+            if (h->cutmode != MCX_CUT_SWITCH) throw mcx_error(MCX_ERR_UNSUPPORTED, "batched gradients need SWITCH mode");
+            if (!h->have_excl || !h->have_coords || !h->have_atoms) throw mcx_error(MCX_ERR_STATE, "system not set up");
             if (!h->list_valid) build_list(h);
             HIPCHK(hipStreamSynchronize(h->stream));
             d[k] = h->sys();
-            gb[k].alloc((size_t)6 * h->n);
+            gb[k].alloc((size_t)8 * h->n);
             ptrs[k] = gb[k].p;
             nmax = std::max(nmax, h->n);
         }
@@ -1402,7 +1483,7 @@ int mcx_time_kernel(mcx_ctx *h, int which, int reps, double *ms)
             DBuf<double> &a;
             ~Release() { a.release(); }
         } rel{g6};
-        if (which == 4) g6.alloc((size_t)6 * h->n);
+        if (which == 4) g6.alloc((size_t)8 * h->n);
         auto launch = [&] {
             switch (which) {
             case 0: launch_full_energy(s, h->tiles.p, h->ntiles, kFullTile, h->partial.p, h->stream); break;

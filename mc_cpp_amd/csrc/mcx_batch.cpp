@@ -406,7 +406,7 @@ int mcx_batch_create(mcx_ctx *const *reps, int nrep, const mcx_move_type *types,
             b->types[r].assign(types + at, types + at + ntypes[r]);
             at += ntypes[r];
             for (const auto &t : b->types[r])
-                if (t.type == MCX_MV_FBIAS)
+                if (t.type == MCX_MV_FBIAS || t.type == MCX_MV_FBIAS_ACC)
                     throw mcx_error(MCX_ERR_UNSUPPORTED, "force-bias moves run in mcx_mc_run only");
         }
         b->mbufs.resize(nrep);

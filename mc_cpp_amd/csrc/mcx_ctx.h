@@ -73,7 +73,8 @@ struct mcx_ctx {
     DBuf<TileAtom> tq;
     DBuf<double4> qes;
     std::vector<double> hx, hy, hz;   // host copy of the last uploaded coordinates
-    DBuf<int> frozen_d, excl_off_d, excl_idx_d, xfull_off_d, xfull_idx_d;
+    bool host_stale = false;          // the device coordinates were changed in place since
+    DBuf<int> frozen_d, excl_off_d, excl_idx_d, xfull_off_d, xfull_idx_d, p14_off_d, p14_idx_d;
     DBuf<double4> cpos;
     DBuf<int> nbr, cnt, half, overflow;
     int cap = 0;
@@ -144,6 +145,7 @@ struct mcx_ctx {
         s.frozen = frozen_d.p;
         s.excl_off = excl_off_d.p; s.excl_idx = excl_idx_d.p;
         s.xfull_off = xfull_off_d.p; s.xfull_idx = xfull_idx_d.p;
+        s.p14_off = p14_off_d.p; s.p14_idx = p14_idx_d.p;
         s.cap = cap;
         s.nbr = nbr.p; s.cnt = cnt.p; s.half = half.p; s.overflow = overflow.p;
         return s;
@@ -185,7 +187,7 @@ struct mcx_ctx {
         adj_par.release();
         adj_type.release();
         mvb.release();
-        DBuf<int> *ib[] = {&frozen_d, &excl_off_d, &excl_idx_d, &xfull_off_d, &xfull_idx_d, &nbr, &cnt, &half, &overflow, &cell_of,
+        DBuf<int> *ib[] = {&frozen_d, &excl_off_d, &excl_idx_d, &xfull_off_d, &xfull_idx_d, &p14_off_d, &p14_idx_d, &nbr, &cnt, &half, &overflow, &cell_of,
                            &cell_start, &cell_count, &cell_atoms, &cursor, &group_start, &cell_tmp, &adj_off, &adj_kk,
                            &cell_acc};
         for (auto *b : ib) b->release();

@@ -77,6 +77,9 @@ struct DevSys {
     // excluded partners of each atom in both directions (pair (a,b), a<b,
     // is excluded iff b is in the visible row of a), CSR
     const int *xfull_off, *xfull_idx;
+    // 1-4 partners of each atom in both directions (one entry per 1-4 pair
+    // record), CSR
+    const int *p14_off, *p14_idx;
     // neighbour list: full rows (all j != i, ascending) in ELL layout
     int cap;
     int *nbr;            // [n*cap]
@@ -106,6 +109,10 @@ struct DevMoves {
     double dmax0[kMaxMoveTypes];
     double target[kMaxMoveTypes];
     int each[kMaxMoveTypes];
+    // force-bias drift beta A F per type: F in iu/A times 1/418.4 (the drift
+    // in kcal/mol/A, MCX_MV_FBIAS) or times 1 (the Metropolis test's own
+    // beta applied to the iu force, MCX_MV_FBIAS_ACC)
+    double fb_scale[kMaxMoveTypes];
     const int4 *inst;                  // flat instance table: (first, last, pivot, 0)
     const int2 *inst_adj;              // range of the instance's term records
     const int2 *inst_pairs;            // range of the instance's pair corrections in pair_ent
@@ -272,9 +279,15 @@ hipError_t launch_mc_wide(const DevSys &s, const DevTerms &t, const DevMoves &mv
 // workgroups of k_mc_wide that can be resident at once on the device
 int mc_wide_max_workgroups(int device);
 // adjoint.hip
-void launch_nonbonded_grad(const DevSys &s, double *g6, hipStream_t st);
-void launch_nonbonded_grad_batch(const DevSys *sys, double *const *g6, int nsys, int nmax, hipStream_t st);
-void launch_type_sums(const double *g6, int n, const int *type, int ntypes, double *out, hipStream_t st);
+// adjoint.hip: g8 = 8 n values per system (dE/dx, dE/dy, dE/dz, dE/deps,
+// dE/dsig, dE/dq, dE/deps14, dE/dsig14); type sums: 5 per type (eps, sig,
+// q, eps14, sig14)
+void launch_nonbonded_grad(const DevSys &s, double *g8, hipStream_t st);
+void launch_nonbonded_grad_batch(const DevSys *sys, double *const *g8, int nsys, int nmax, hipStream_t st);
+void launch_type_sums(const double *g8, int n, const int *type, int ntypes, double *out, hipStream_t st);
+// mc.hip: the fixed-point mirror recomputed from the device coordinates
+// (charge bits kept)
+void launch_mirror_from_coords(const DevSys &s, hipStream_t st);
 // batch.hip: one Metropolis chain per replica, one workgroup each
 constexpr int kBatchTypes = 64;     // distinct (q, eps, sig) atom types of a replica
 constexpr int kBatchAtoms = 32;     // atoms per move instance

@@ -100,6 +100,8 @@ SIGNATURES = {
     "mcx_nonbonded_grad_types": ([C.c_void_p, C.c_int, _ip] + [_dp] * 6, C.c_int),
     "mcx_nonbonded_grad_batch": ([C.POINTER(C.c_void_p), C.c_int, C.POINTER(_dp)], C.c_int),
     "mcx_device_coords": ([C.c_void_p, C.c_int, C.POINTER(C.c_void_p)], C.c_int),
+    "mcx_coords_updated": ([C.c_void_p], C.c_int),
+    "mcx_nonbonded_grad_device": ([C.c_void_p, C.c_void_p, C.c_int, _ip, C.c_void_p], C.c_int),
     "mcx_time_kernel": ([C.c_void_p, C.c_int, C.c_int, _dp], C.c_int),
     "mcx_set_rng_stream": ([C.c_void_p, C.c_uint32], C.c_int),
     "mcx_batch_create": ([C.POINTER(C.c_void_p), C.c_int, C.POINTER(MoveTypeC), _ip, C.POINTER(C.c_void_p)],
@@ -287,6 +289,26 @@ class Handle:
         self._chk(self.L.mcx_nonbonded_grad_types(self.h, nt, _p(ty, C.c_int), *[_p(a, C.c_double) for a in
                                                                                 (gx, gy, gz, ge, gs, gq)]))
         return gx, gy, gz, ge, gs, gq
+
+    def device_coords(self, which) -> int:
+        """Device address of the master coordinate array x (0), y (1), z (2)."""
+        p = C.c_void_p(0)
+        self._chk(self.L.mcx_device_coords(self.h, int(which), C.byref(p)))
+        return int(p.value)
+
+    def coords_updated(self):
+        """The device coordinate arrays were written in place: refresh the
+        fixed-point mirror on the device, mark the list stale."""
+        self._chk(self.L.mcx_coords_updated(self.h))
+
+    def grad_device(self, gpos_ptr, type_of_atom=None, ntypes=0, gtype_ptr=0):
+        """Adjoint into device memory: gpos_ptr -> 3 n doubles (dE/dx, dE/dy,
+        dE/dz); with types, gtype_ptr -> 5 ntypes doubles per type (eps,
+        sig, q, eps14, sig14)."""
+        ty = _i(type_of_atom) if type_of_atom is not None else None
+        self._chk(self.L.mcx_nonbonded_grad_device(self.h, C.c_void_p(int(gpos_ptr)), int(ntypes),
+                                                   _p(ty, C.c_int) if ty is not None else None,
+                                                   C.c_void_p(int(gtype_ptr)) if gtype_ptr else None))
 
     def time_kernel(self, which, reps=20):
         ms = C.c_double(0)

@@ -26,6 +26,7 @@ RBOLTZUI = 0.83144621     # include/Constants.hpp:35
 PBC_NONE, PBC_CUBIC, PBC_ORBIC = 0, 1, 2
 CUT_FULL, CUT_SWITCH = 0, 1
 MV_TRN, MV_ROT, MV_FBIAS = 1, 2, 3   # MV_FBIAS: force-bias translation (not in the reference)
+MV_FBIAS_ACC = 4   # force-bias with the drift at the Metropolis test's own beta (not in the reference)
 MODE_RESIDUE, MODE_ALL, MODE_ATOM = 1, 2, 4
 
 

@@ -1024,3 +1024,96 @@ void orc_nonbonded_switch_grad(orc_system *s, double *gx, double *gy, double *gz
         }
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* Gradient of the whole nonbonded energy elec + vdw, FP64.  Not in the  */
+/* reference; the derivative of computeNonBonded_switch (:336-424) with  */
+/* computeNonBonded14_switch (:426-488) in SWITCH mode, and of           */
+/* computeNonBonded_full (:194-265) with computeNonBonded14 (:267-318)   */
+/* in FULL mode.  g8 = 8 n values: dE/dx, dE/dy, dE/dz, dE/deps, dE/dsig, */
+/* dE/dq, dE/deps14, dE/dsig14 per atom (parameters in the AtomList      */
+/* convention).                                                          */
+/* ------------------------------------------------------------------ */
+
+/* one pair's C q_i q_j / r + 4 e_i e_j (u^12 - u^6), u = (s_i + s_j) / r,
+   times the switch when `sw`; its gradient added to both atoms, the LJ
+   parameter parts to components ce (eps) and cs (sig) */
+static void pair_grad(const orc_system *s, int i, int j, const double d[3], double r2, double ei, double ej,
+                      double si, double sj, int sw, double ctoff2, double cton2, double switch2,
+                      double *g8, int ce, int cs)
+{
+    const int N = s->natom;
+    const double C = CHGCHARMM * KCALTOIU;
+    double r = sqrt(r2), rinv = 1.0 / r;
+    double sij = si + sj;
+    double u = sij * rinv, u6 = X6(u), u12 = u6 * u6;
+    double eij = ei * ej;
+    double qq = s->charge[i] * s->charge[j];
+    double Ee = C * qq * rinv;
+    double Ev = 4.0 * eij * (u12 - u6);
+    double S = 1.0, dSdr2 = 0.0;
+    if (sw && r2 > cton2) {
+        double a = ctoff2 - r2, b = ctoff2 + 2. * r2 - 3. * cton2;
+        S = a * a * b * switch2;
+        dSdr2 = (-2.0 * a * b + 2.0 * a * a) * switch2;
+    }
+    double g = S * (-0.5 * Ee / r2 + 4.0 * eij * (-6.0 * u12 + 3.0 * u6) / r2) + dSdr2 * (Ee + Ev);
+    for (int c = 0; c < 3; c++) {          /* d = x_j - x_i: dr2/dx_j = 2d, dr2/dx_i = -2d */
+        g8[(size_t)c * N + j] += 2.0 * g * d[c];
+        g8[(size_t)c * N + i] -= 2.0 * g * d[c];
+    }
+    double dv_deij = S * 4.0 * (u12 - u6);
+    g8[(size_t)ce * N + i] += dv_deij * ej;
+    g8[(size_t)ce * N + j] += dv_deij * ei;
+    if (sij != 0.0) {
+        double dv_dsij = S * 4.0 * eij * (12.0 * u12 - 6.0 * u6) / sij;
+        g8[(size_t)cs * N + i] += dv_dsij;
+        g8[(size_t)cs * N + j] += dv_dsij;
+    }
+    double de_dqq = S * C * rinv;
+    g8[5 * (size_t)N + i] += de_dqq * s->charge[j];
+    g8[5 * (size_t)N + j] += de_dqq * s->charge[i];
+}
+
+void orc_nonbonded_grad(orc_system *s, double *g8)
+{
+    const int N = s->natom;
+    const double ctoff2 = s->ctoff * s->ctoff, cton2 = s->cton * s->cton;
+    const double switch2 = 1. / (X3(ctoff2 - cton2));
+    const int sw = s->cutmode == ORC_CUT_SWITCH;
+    for (size_t k = 0; k < 8 * (size_t)N; k++) g8[k] = 0.0;
+    double di[3], dj[3], d[3];
+    if (sw) {
+        for (int i = 0; i < N; i++) {
+            getc3(s, i, di);
+            const int *row = s->neighList + (size_t)i * s->cap;
+            for (int k = 0; k < s->neighPair[i]; k++) {
+                int j = row[k];
+                getc3(s, j, dj);
+                double r2 = distance2v(s, di, dj, d);
+                if (r2 > ctoff2) continue;
+                pair_grad(s, i, j, d, r2, s->eps[i], s->eps[j], s->sig[i], s->sig[j], 1, ctoff2, cton2, switch2,
+                          g8, 3, 4);
+            }
+        }
+    } else {
+        for (int i = 0; i < N - 1; i++) {
+            getc3(s, i, di);
+            for (int j = i + 1; j < N; j++) {
+                if (excl_member(s, i, j)) continue;
+                getc3(s, j, dj);
+                double r2 = distance2v(s, di, dj, d);
+                pair_grad(s, i, j, d, r2, s->eps[i], s->eps[j], s->sig[i], s->sig[j], 0, ctoff2, cton2, switch2,
+                          g8, 3, 4);
+            }
+        }
+    }
+    for (int k = 0; k < s->nPair14; k++) {
+        int i = s->neighList14[2 * k], j = s->neighList14[2 * k + 1];
+        getc3(s, i, di); getc3(s, j, dj);
+        double r2 = distance2v(s, di, dj, d);
+        if (sw && r2 > ctoff2) continue;
+        pair_grad(s, i, j, d, r2, s->eps14[i], s->eps14[j], s->sig14[i], s->sig14[j], sw, ctoff2, cton2, switch2,
+                  g8, 6, 7);
+    }
+}

@@ -128,6 +128,9 @@ double orc_mc_run(orc_system *s, const orc_movetype *types, int ntypes,
 
 /* FP64 reference gradients of the nonbonded switched energy:
  * dE/dx,dy,dz [natom] and dE/deps, dE/dsig, dE/dq per atom (iu). */
+/* dE/dx, dE/dy, dE/dz, dE/deps, dE/dsig, dE/dq, dE/deps14, dE/dsig14 (8 n
+   values) of elec + vdw, list and 1-4 pairs, SWITCH or FULL mode */
+void orc_nonbonded_grad(orc_system *s, double *g8);
 void orc_nonbonded_switch_grad(orc_system *s, double *gx, double *gy, double *gz,
                                double *geps, double *gsig, double *gq);
 

@@ -100,6 +100,7 @@ def lib():
         L.orc_rint.argtypes = [C.c_double]
         L.orc_rint.restype = C.c_double
         L.orc_nonbonded_switch_grad.argtypes = [P] + [_dp] * 6
+        L.orc_nonbonded_grad.argtypes = [P, _dp]
         L.orc_set_energy_threads.argtypes = [C.c_int]
         for f in ("orc_nonbonded_full", "orc_nonbonded_switch", "orc_nonbonded14", "orc_nonbonded14_switch",
                   "orc_ebond", "orc_eang", "orc_eub", "orc_edihe", "orc_eimpr"):
@@ -217,6 +218,14 @@ class Oracle:
         out = [np.zeros(n) for _ in range(6)]
         self.L.orc_nonbonded_switch_grad(C.byref(self.s), *[_p(a, C.c_double) for a in out])
         return out
+
+    def grad_all(self):
+        """(8, n): dE/dx, dE/dy, dE/dz, dE/deps, dE/dsig, dE/dq, dE/deps14,
+        dE/dsig14 of elec + vdw (list or all pairs, and 1-4 pairs)."""
+        n = self.sys.natom
+        g = np.zeros(8 * n)
+        self.L.orc_nonbonded_grad(C.byref(self.s), _p(g, C.c_double))
+        return g.reshape(8, n)
 
     # ---- Monte Carlo ----
     def mc_run(self, nsteps, seed, mode=0, list_every=50, moves=None, nthreads=None):

@@ -1,10 +1,11 @@
 """Force-bias (smart Monte Carlo) translations of the device loop against a
 host replay built on the FP64 oracle.
 
-The drift uses beta = 1/(R T) in mol/kcal and F in kcal/mol/A; the
-acceptance test applies the same beta to dE in iu, as the reference's test
-does for every move, so force-bias and TRN/ROT moves leave one density
-invariant (the Metropolis-Hastings ratio holds for any drift).
+The drift uses beta = 1/(R T) in mol/kcal and F in kcal/mol/A (MV_FBIAS),
+or F in iu/A (MV_FBIAS_ACC: the drift of the density the test samples); the
+acceptance test applies beta to dE in iu, as the reference's test does for
+every move, so force-bias and TRN/ROT moves leave one density invariant (the
+Metropolis-Hastings ratio holds for any drift).
 
 Force-bias moves are not in the reference; they follow Rossky, Doll and
 Friedman, J. Chem. Phys. 69, 4628 (1978): a moved set's displacement is
@@ -47,9 +48,9 @@ def _box(seed=5, nside=8, L=24.85):
     return s
 
 
-def _fb_move(s, A):
+def _fb_move(s, A, typ=systems.MV_FBIAS):
     sele = (np.arange(s.natom) % 3 == 0).astype(np.int32)     # the oxygen of each water
-    return systems.MoveType(systems.MV_FBIAS, systems.MODE_RESIDUE, A, 50.0, 0, sele)
+    return systems.MoveType(typ, systems.MODE_RESIDUE, A, 50.0, 0, sele)
 
 
 def _wrap(v, L):
@@ -63,6 +64,8 @@ def fb_replay(s, move, nsteps, seed):
     mt = MT19937(seed)
     beta = 1.0 / (KB_REF * s.temp)
     A = move.dmax
+    # drift force in kcal/mol/A (FBIAS) or in iu/A (FBIAS_ACC: the test's own beta)
+    fscale = 1.0 if move.type == systems.MV_FBIAS_ACC else 1.0 / KCALTOIU
     atoms = np.nonzero(move.sele)[0]
     out = dict(inst=[], disp=[], de=[], lnq=[], lnq_scale=[], acc=[])
 
@@ -82,14 +85,14 @@ def fb_replay(s, move, nsteps, seed):
                        b * math.cos(2 * math.pi * u[3])])
         o.update_list()
         e0 = o.getE()[0]
-        f_old = net_force(idx) / KCALTOIU
+        f_old = net_force(idx) * fscale
         disp = beta * A * f_old + math.sqrt(2.0 * A) * xi
         old = [c[idx].copy() for c in (s.x, s.y, s.z)]
         for c, arr in enumerate((s.x, s.y, s.z)):
             arr[idx] = [_wrap(v + disp[c], s.box[c]) for v in old[c]]
         o.update_list()
         e1 = o.getE()[0]
-        f_new = net_force(idx) / KCALTOIU
+        f_new = net_force(idx) * fscale
         de = e1 - e0
         fw = disp - beta * A * f_old
         rv = -disp - beta * A * f_new
@@ -107,11 +110,13 @@ def fb_replay(s, move, nsteps, seed):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("A", [1e-3, 1e-2])
-def test_force_bias_replay_matches_host(A):
+@pytest.mark.parametrize("typ,A", [(systems.MV_FBIAS, 1e-3), (systems.MV_FBIAS, 1e-2), (systems.MV_FBIAS_ACC, 5e-6)])
+def test_force_bias_replay_matches_host(typ, A):
+    """FBIAS_ACC: the drift at the test's own beta (F in iu/A), A ~418x
+    smaller for drifts of the same size."""
     s = _box()
     ref = s.copy()
-    move = _fb_move(s, A)
+    move = _fb_move(s, A, typ)
     n, seed = 150, 2024
     ff = mcx.FField(s)
     res = ff.h.mc_run([move], n, seed, rng_mode=mcx.MC_REPLAY, list_every=10)
@@ -124,9 +129,15 @@ def test_force_bias_replay_matches_host(A):
             (k, res["lnq"][k], host["lnq"][k])
         assert abs(res["de"][k] - host["de"][k]) <= 1e-5 * max(abs(host["de"][k]), kT), (k, res["de"][k], host["de"][k])
         assert res["acc"][k] == host["acc"][k], k
-    # the acceptance uniform was drawn (negative exponent) and both outcomes occur
+    # the acceptance uniform was drawn (negative exponent) and both outcomes
+    # occur (with the test's own beta the drift follows the force closely
+    # enough that this relaxing lattice accepts nearly every move; larger A
+    # carries molecules beyond the list skin between the rebuilds)
     x = res["lnq"] - res["de"] / (KB_REF * s.temp)
-    assert np.any(x < 0.0) and 0 < int(res["acc"].sum()) < n
+    if typ == systems.MV_FBIAS:
+        assert np.any(x < 0.0) and 0 < int(res["acc"].sum()) < n
+    else:
+        assert int(res["acc"].sum()) > n // 2
     x, y, z = ff.get_coords()
     assert np.max(np.abs(x - ref.x)) <= 1e-6 and np.max(np.abs(z - ref.z)) <= 1e-6
     e_fresh = ff.h.energy()[0]

@@ -476,3 +476,50 @@ def test_full_size_replay_fixture_prefix(oracle_lib):
     assert np.array_equal(res["mvinst"], g["mvinst"][:200])
     assert np.array_equal(res["acc"], g["acc"][:200])
     assert res["de"].tolist() == g["de"][:200].tolist()
+
+
+@pytest.mark.parametrize("cutmode", [systems.CUT_SWITCH, systems.CUT_FULL])
+def test_full_gradient_with_14_pairs_vs_central_differences(oracle_lib, cutmode):
+    """orc_nonbonded_grad -- dE/d(x, y, z, eps, sig, q, eps14, sig14) of
+    elec + vdw with list (or all) pairs and 1-4 pairs -- against central
+    differences of the oracle's own energy, on a bonded system (chains with
+    1-4 pairs among waters, orthorhombic box); and equal to the list-pair
+    gradient where a system has no 1-4 pairs."""
+    s, _ = _small_bonded_mix(cutmode)
+    o = O.Oracle(s)
+    if cutmode == systems.CUT_SWITCH:
+        o.update_list()
+    assert len(o.pairs14()) > 0
+    g = o.grad_all()
+
+    def enb():
+        e = o.getE()
+        return e[3] + e[4]
+
+    nw = s.natom - 36                     # the chain beads come last
+    atoms = [nw, nw + 5, nw + 13, 0, 4]
+    arrays = {0: s.x, 1: s.y, 2: s.z, 3: s.eps, 4: s.sig, 5: s.charge, 6: s.eps14, 7: s.sig14}
+    checked = 0
+    for c, arr in arrays.items():
+        for i in atoms:
+            v0 = arr[i]
+            hh = 1e-5 if c < 3 else 1e-4 * max(abs(v0), 0.1)
+            arr[i] = v0 + hh
+            ep = enb()
+            arr[i] = v0 - hh
+            em = enb()
+            arr[i] = v0
+            fd = (ep - em) / (2 * hh)
+            assert g[c, i] == pytest.approx(fd, rel=1e-5, abs=1e-4 * max(1.0, abs(fd))), (c, i, g[c, i], fd)
+            checked += abs(fd) > 0
+    assert checked >= 20
+    # no 1-4 pairs: the list-pair gradient of orc_nonbonded_switch_grad
+    if cutmode == systems.CUT_SWITCH:
+        w, _ = systems.water_box(seed=6, nside=5, L=15.53)
+        ow = O.Oracle(w)
+        ow.update_list()
+        ga = ow.grad_all()
+        gs = ow.grad_switch()
+        for c in range(6):
+            assert np.allclose(ga[c], gs[c], rtol=1e-12, atol=1e-9 * np.abs(gs[c]).max())
+        assert not ga[6:].any()
