@@ -273,3 +273,76 @@ def test_tuner_keeps_state():
         assert np.array_equal(h.get_coords()[0], c[0])
     out = b.run(2)
     assert out["moves"] == 2 * sum(s.natom // 3 for s in reps)
+
+
+def test_ladder_256_replicas_full_size():
+    """The benchmark's ladder at full size: 256 water boxes of 306-2,986
+    molecules (872,904 atoms), T 280-400 K geometric, one batch.
+      * 300 Philox moves of every replica; six replicas (the smallest, the
+        largest and four across the ladder) against the single-chain
+        device loop on the same stream and temperature: same accept
+        sequence, dE within 1e-9 of max(|dE|, kT) below 5e4 iu;
+      * four sweeps each followed by an exchange round, every round's
+        decisions recomputed on the host;
+      * every replica's running total against a fresh device energy of its
+        final coordinates: 2e-6 of the larger of its start and final totals
+        (the fresh totals sum Lennard-Jones terms in FP32, and the lattice
+        starts hold near-contacts whose terms reach ~1e6 iu);
+      * for the six sampled replicas, the chain's energy change (the sum of
+        its accepted dE) against the FP64 oracle's E(final) - E(start):
+        1e-6 of the larger total."""
+    reps = systems.water_replicas(seed=4, nrep=256)
+    sizes = np.array([s.natom for s, _ in reps])
+    temps = [s.temp for s, _ in reps]
+    sample = sorted({int(np.argmin(sizes)), int(np.argmax(sizes)), 40, 100, 170, 230})
+    keep = {r: reps[r][0].copy() for r in sample}
+    handles = [mcx.from_system(s) for s, _ in reps]
+    b = mcx.Batch(handles, reps[0][0].moves)
+    b.set_ladder(temps)
+    seed, n = 2024, 300
+    b.reset(seed, mcx.MC_PHILOX)
+    b0 = b.state()["etot"].copy()
+    e_start = np.abs(b0)
+    acc, de = b.moves(n)
+    for r in sample:
+        s = keep[r]
+        ff = mcx.FField(s)
+        ff.h.set_rng_stream(r)
+        res = ff.h.mc_run(s.moves, n, seed, rng_mode=mcx.MC_PHILOX, list_every=50)
+        n_ok = _compare(acc[r], de[r], res["acc"], res["de"], _tol(s, 1e-9), "replica %d" % r, DE_CMP)
+        assert n_ok == n, ("replica %d diverged at move %d" % (r, n_ok))
+    # exchange rounds (the round counter runs from the reset)
+    beta = [1.0 / (KB_REF * t) for t in temps]
+    slot = b.state()["slot"].copy()
+    assert np.array_equal(slot, np.arange(256))
+    for rnd in range(4):
+        out = b.run(1, exchange_every=1, use_graph=False)
+        assert out["exchange_attempts"] == len(range(rnd & 1, 255, 2))
+        st = b.state()
+        e = st["etot"]
+        rep_of_slot = np.argsort(slot)
+        expect = slot.copy()
+        for k in range(rnd & 1, 255, 2):
+            ra, rb = rep_of_slot[k], rep_of_slot[k + 1]
+            delta = (beta[k] - beta[k + 1]) * (e[ra] - e[rb])
+            if delta >= 0.0 or _exchange_u(seed, k, rnd) < math.exp(delta):
+                expect[ra], expect[rb] = k + 1, k
+        assert np.array_equal(st["slot"], expect), rnd
+        slot = expect
+    assert not np.array_equal(slot, np.arange(256))
+    st = b.state()
+    for r, h in enumerate(handles):
+        e_fresh = h.energy()[0]
+        assert abs(st["etot"][r] - e_fresh) <= 2e-6 * max(abs(e_fresh), e_start[r]), (r, st["etot"][r], e_fresh)
+    for r in sample:
+        s0 = reps[r][0].copy()
+        o0 = Oracle(s0)
+        o0.update_list(16)
+        e0 = o0.getE()[0]
+        s1 = reps[r][0].copy()
+        s1.x[:], s1.y[:], s1.z[:] = handles[r].get_coords()
+        o1 = Oracle(s1)
+        o1.update_list(16)
+        e1 = o1.getE()[0]
+        change = st["etot"][r] - b0[r]
+        assert abs(change - (e1 - e0)) <= 1e-6 * max(abs(e0), abs(e1)), (r, change, e1 - e0)
