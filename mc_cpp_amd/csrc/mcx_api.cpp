@@ -1207,7 +1207,10 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
         // after it bail out)
         bool full_next = false;
         auto check_bail = [&]() {
-            check_wide();
+            // a launch whose workgroups were not resident together changed
+            // nothing and only handed its chunk back: no rebuild for it (a
+            // list overflow still shows in the flag and bails the relaunch)
+            const bool retry = check_wide();
             if (!a.bail) return;
             int b = kNone, nst = 0;
             HIPCHK(hipMemcpyAsync(&b, bail_d.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
@@ -1217,7 +1220,7 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
             if (nst > kPatchStale / 2) full_next = true;
             if (b != kNone) {
                 done = b;
-                if (rebuilds) {
+                if (rebuilds && !retry) {
                     build_list(h);           // grows the capacity until the rows fit
                     nlist++;
                 }
