@@ -432,14 +432,17 @@ def main():
                  "work": "per move: %.2f moved atoms x %.1f full-row candidates x 2 (old, new) x "
                          "(17 + 23 x f_in), f_in = %.4f in-cutoff fraction of the list" % (atoms_per_move, row_mean, f_in)}
 
-    # ---- end-to-end getE as the drop-in FField does it: upload + energy ----
+    # ---- end-to-end getE as the drop-in FField_MCX does it: the coordinates
+    # the move code changed are uploaded, the energy summed over the list of
+    # the last update (mcx_set_coords_moved + mcx_energy) ----
     x, y, z = h.get_coords()
-    h.set_coords(x, y, z)
+    h.update_list()
+    h.set_coords_moved(x, y, z)
     h.energy()
-    reps_e = 50
+    reps_e = 200
     t0 = time.perf_counter()
     for _ in range(reps_e):
-        h.set_coords(x, y, z)
+        h.set_coords_moved(x, y, z)
         h.energy()
     getE_ms = (time.perf_counter() - t0) / reps_e * 1e3
 
@@ -528,8 +531,9 @@ def main():
         "pair_evals_per_s": pairs / (ms_full * 1e-3),
         "full_energy_ms": ms_full,
         "switch_energy_ms": ms_sw,
-        "getE_end_to_end": {"ms": getE_ms, "what": "mcx_set_coords from host arrays + mcx_energy (the drop-in "
-                                                   "FField_MCX::getE), wall clock, PCIe included",
+        "getE_end_to_end": {"ms": getE_ms, "what": "mcx_set_coords_moved from host arrays + mcx_energy over the "
+                                                   "list of the last update (the drop-in FField_MCX::getE), wall "
+                                                   "clock, PCIe included",
                             "reference_cpu_ms": cpu.get("reference_getE_ms") if cpu else None},
         "batched": batch,
         "force_bias": fbias,

@@ -103,8 +103,17 @@ int mcx_build_exclusions(mcx_ctx *h);
 int mcx_get_exclusions(mcx_ctx *h, int *exclPair, int *idx, long long cap, long long *nidx,
                        int *pairs14, int cap14, int *npair14);
 
-/* AtomList::setCoords / getCoords for all atoms (src/FORTRAN_interface.cpp:77-82). */
+/* AtomList::setCoords / getCoords for all atoms (src/FORTRAN_interface.cpp:77-82).
+ * mcx_set_coords marks the pair list stale (the next energy rebuilds it). */
 int mcx_set_coords(mcx_ctx *h, const double *x, const double *y, const double *z);
+
+/* Coordinates changed by Monte Carlo moves, as MOVE_TRN / MOVE_ROT change
+ * AtomList in place (include/Move.hpp:42-143): the pair list stays the one
+ * built at the last update, as the reference's Verlet list does until
+ * askListUpdate (src/FField.cpp:222-235), so a following mcx_energy is the
+ * reference's getE over that list.  The fixed-point mirror is recomputed on
+ * the device.  The first upload of a handle behaves as mcx_set_coords. */
+int mcx_set_coords_moved(mcx_ctx *h, const double *x, const double *y, const double *z);
 int mcx_get_coords(mcx_ctx *h, double *x, double *y, double *z);
 
 /* List_nonBonded::update_verlet_list (src/List_nonBonded.cpp:836-915) on the

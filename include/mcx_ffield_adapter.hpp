@@ -6,8 +6,9 @@
 // (src/Parser.cpp:139-153); after IO_MDBAS has filled the atom and bonded
 // lists and List_nonBonded was constructed, call uploadTopology() once.
 // Every getE() then uploads the coordinates the move code mutated in place
-// (include/Move.hpp:42-143) and evaluates the energy on the GPU;
-// askListUpdate() rebuilds the pair list on the GPU.  runMetropolis() is the
+// (include/Move.hpp:42-143) and evaluates the energy on the GPU over the
+// pair list of the last update; askListUpdate() rebuilds the list on the
+// GPU.  runMetropolis() is the
 // device-resident alternative to MC_metropolis::run.
 //
 // Interfaces implemented: include/FField.hpp:52-153 (public getE,
@@ -128,11 +129,13 @@ private:
     {
         if (rc != MCX_OK) throw std::runtime_error(std::string(what) + ": " + mcx_last_error(h_));
     }
+    // the coordinates the move code changed; the pair list stays the one of
+    // the last askListUpdate, as the reference's does
     void pushCoords()
     {
         x_.resize(n_); y_.resize(n_); z_.resize(n_);
         for (int i = 0; i < n_; i++) { x_[i] = at_List.getX(i); y_[i] = at_List.getY(i); z_[i] = at_List.getZ(i); }
-        check(mcx_set_coords(h_, x_.data(), y_.data(), z_.data()), "mcx_set_coords");
+        check(mcx_set_coords_moved(h_, x_.data(), y_.data(), z_.data()), "mcx_set_coords_moved");
     }
     template <typename T>
     void torsions(const std::vector<T> &v, bool improper)

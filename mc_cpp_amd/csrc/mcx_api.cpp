@@ -854,6 +854,27 @@ int mcx_set_coords(mcx_ctx *h, const double *x, const double *y, const double *z
     });
 }
 
+int mcx_set_coords_moved(mcx_ctx *h, const double *x, const double *y, const double *z)
+{
+    return guarded(h, [&] {
+        if (!x || !y || !z) throw mcx_error(MCX_ERR_ARG, "null coordinate array");
+        if (!h->have_coords) {
+            upload_coords(h, x, y, z);
+            return;
+        }
+        const int n = h->n;
+        h->x.upload(x, n, h->stream);
+        h->y.upload(y, n, h->stream);
+        h->z.upload(z, n, h->stream);
+        h->hx.assign(x, x + n); h->hy.assign(y, y + n); h->hz.assign(z, z + n);
+        h->host_stale = false;
+        launch_mirror_from_coords(h->sys(), h->stream);
+        HIPCHK(hipGetLastError());
+        // the moves since the last list update were not recorded on the device
+        h->patch_valid = false;
+    });
+}
+
 int mcx_get_coords(mcx_ctx *h, double *x, double *y, double *z)
 {
     return guarded(h, [&] {

@@ -101,6 +101,7 @@ SIGNATURES = {
     "mcx_nonbonded_grad_batch": ([C.POINTER(C.c_void_p), C.c_int, C.POINTER(_dp)], C.c_int),
     "mcx_device_coords": ([C.c_void_p, C.c_int, C.POINTER(C.c_void_p)], C.c_int),
     "mcx_coords_updated": ([C.c_void_p], C.c_int),
+    "mcx_set_coords_moved": ([C.c_void_p, _dp, _dp, _dp], C.c_int),
     "mcx_nonbonded_grad_device": ([C.c_void_p, C.c_void_p, C.c_int, _ip, C.c_void_p], C.c_int),
     "mcx_time_kernel": ([C.c_void_p, C.c_int, C.c_int, _dp], C.c_int),
     "mcx_set_rng_stream": ([C.c_void_p, C.c_uint32], C.c_int),
@@ -218,6 +219,12 @@ class Handle:
     def set_coords(self, x, y, z):
         x, y, z = _d(x), _d(y), _d(z)
         self._chk(self.L.mcx_set_coords(self.h, _p(x, C.c_double), _p(y, C.c_double), _p(z, C.c_double)))
+
+    def set_coords_moved(self, x, y, z):
+        """Coordinates changed by moves: the pair list stays as built (the
+        reference's getE between list updates)."""
+        x, y, z = _d(x), _d(y), _d(z)
+        self._chk(self.L.mcx_set_coords_moved(self.h, _p(x, C.c_double), _p(y, C.c_double), _p(z, C.c_double)))
 
     def get_coords(self):
         x, y, z = np.zeros(self.n), np.zeros(self.n), np.zeros(self.n)

@@ -626,3 +626,33 @@ def test_mc_replay_water12288_10k_against_fixture():
     x, y, z = ff.get_coords()
     p = g["pick"]
     assert np.max(np.abs(np.stack([x[p], y[p], z[p]], 1) - g["xyz"])) < 1e-9
+
+
+def test_energy_over_the_list_of_the_last_update():
+    """mcx_set_coords_moved (what the drop-in getE uploads after the
+    reference's move code changed AtomList) keeps the pair list of the last
+    update, as the reference's getE does between askListUpdate calls: on
+    the 12,288-atom box with ten pairs of distant molecules swapped (each
+    carried far beyond the 2 A skin, no overlaps) the energy is the FP64
+    energy over the old list; mcx_set_coords rebuilds, and the two differ."""
+    s, _ = systems.water_box(seed=2)
+    o, ref = _pair(s)
+    o.update_list(16)
+    ff = mcx.FField(s)                   # its list: built at these coordinates
+    rng = np.random.default_rng(5)
+    for m1, m2 in rng.permutation(s.natom // 3)[:20].reshape(10, 2):
+        d = np.array([ref.x[3 * m2] - ref.x[3 * m1], ref.y[3 * m2] - ref.y[3 * m1], ref.z[3 * m2] - ref.z[3 * m1]])
+        for a in range(3):
+            i, j = 3 * m1 + a, 3 * m2 + a
+            ref.x[i] += d[0]; ref.y[i] += d[1]; ref.z[i] += d[2]
+            ref.x[j] -= d[0]; ref.y[j] -= d[1]; ref.z[j] -= d[2]
+    ff.h.set_coords_moved(ref.x, ref.y, ref.z)
+    eg = ff.h.energy()
+    eo = o.getE()                        # the oracle's list is the old one too
+    _check_energy(eg, eo, "list of the last update")
+    ff.h.set_coords(ref.x, ref.y, ref.z)
+    eg2 = ff.h.energy()
+    o.update_list(16)
+    eo2 = o.getE()
+    _check_energy(eg2, eo2, "rebuilt list")
+    assert eo[0] != eo2[0]
