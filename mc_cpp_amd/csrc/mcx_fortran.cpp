@@ -272,7 +272,9 @@ void set_coords_(double x[], double y[], double z[])
 {
     guarded([&] {
         if (!g.h) throw std::runtime_error("set_coords_ before set_ff_and_cor_");
-        check(mcx_set_coords(g.h, x, y, z), "mcx_set_coords");
+        // AtomList::setCoords only: the list built at set-up stays, as the
+        // reference's does (src/FORTRAN_interface.cpp:77-82)
+        check(mcx_set_coords_moved(g.h, x, y, z), "mcx_set_coords_moved");
     });
 }
 

@@ -71,15 +71,17 @@ def test_fortran_energy_of_reference_inputs(name):
     m = np.zeros(gold["natom"])
     L.get_mass_(m.ctypes.data_as(C.POINTER(C.c_double)))
     assert np.array_equal(m, s.mass)
-    # new coordinates: against the oracle's FP64 energy of the same ones
+    # new coordinates: against the oracle's FP64 energy of the same ones,
+    # over the list built at set-up (the reference keeps it: set_coords_ is
+    # AtomList::setCoords only)
+    o = Oracle(s)
+    if cut == "switch":
+        o.update_list()
     rng = np.random.default_rng(5)
     for arr in (s.x, s.y, s.z):
         arr += rng.uniform(-0.05, 0.05, s.natom)
     P = lambda a: np.ascontiguousarray(a).ctypes.data_as(C.POINTER(C.c_double))
     L.set_coords_(P(s.x), P(s.y), P(s.z))
     e = _energy(L)
-    o = Oracle(s)
-    if cut == "switch":
-        o.update_list()
     _check(e, o.getE() / 418.4)
     L.clean_()
