@@ -10,9 +10,11 @@
 //    here and subtracted by the fused bonded kernel, which keeps this loop
 //    free of per-pair branches.
 //  * k_switch_energy: the switched sum over the Verlet list
-//    (computeNonBonded_switch, :336-424).  One wavefront per atom row of
-//    the sorted full list, reading the j > i suffix in chunks whose loads
-//    are all in flight together.
+//    (computeNonBonded_switch, :336-424).  A group of 16, 32 or 64 lanes
+//    per atom row of the sorted full list (consecutive rows share a
+//    wavefront), reading the j > i suffix in steps whose loads are all in
+//    flight together, the next step's indices loaded during this step's
+//    arithmetic.
 //
 // Pair arithmetic: the displacement is the wrapping int32 difference of the
 // fixed-point fractional coordinates (the exact minimum image, no rint),
@@ -23,6 +25,8 @@
 // costs a few issue slots and buys the 1e-6 relative total that water's
 // 700-fold Coulomb cancellation demands.
 #include "terms.h"
+
+#include <cstdlib>
 
 namespace {
 
@@ -85,32 +89,46 @@ __device__ __forceinline__ void tile_pair_acc(const DevSys &s, int i, int j, con
 }
 
 #ifndef MCX_TILE_ILP
-#define MCX_TILE_ILP 2
+#define MCX_TILE_ILP 4
 #endif
 
-// The j atoms of a tile in order.  Off the diagonal, two j atoms go
-// together: their distance chains and 1/r estimates are independent, then
-// each Lennard-Jones branch, then both Newton steps, with separate
-// accumulators so that the two chains overlap.
+// The j atoms of a tile in order.  Off the diagonal, K j atoms go together:
+// their scalar loads are issued at once, their distance chains and 1/r
+// estimates are independent, then each Lennard-Jones branch, then the
+// Newton steps, into two accumulator pairs, so that the chains overlap and
+// a wavefront waits on the scalar loads once per K atoms.
 template <int PB, bool DIAG>
 __device__ __forceinline__ void full_tile(const DevSys &s, int i, int j0, int j1, const int4 pi, double xi, double yi,
                                           double zi, float si, double &accC, double &accV)
 {
+    constexpr int K = MCX_TILE_ILP;
     const int4 *__restrict__ ipos = s.ipos;
     const TileAtom *__restrict__ tq = s.tq;
     int j = j0;
-    if (!DIAG && MCX_TILE_ILP == 2) {
+    if (!DIAG && K > 1) {
         double accC2 = 0.0, accV2 = 0.0;
-        for (; j + 1 < j1; j += 2) {
-            const int4 pa = ipos[j], pb = ipos[j + 1];
-            const TileAtom qa = tq[j], qb = tq[j + 1];
-            const double r2a = mcx_tile_r2<PB>(s, pi, pa, xi, yi, zi, j);
-            const double r2b = mcx_tile_r2<PB>(s, pi, pb, xi, yi, zi, j + 1);
-            const double r0a = __builtin_amdgcn_rsq(r2a), r0b = __builtin_amdgcn_rsq(r2b);
-            if (scalar_nonzero(qa.eps)) accV = fma((double)qa.eps, (double)mcx_tile_lj(si + qa.sig, r0a), accV);
-            if (scalar_nonzero(qb.eps)) accV2 = fma((double)qb.eps, (double)mcx_tile_lj(si + qb.sig, r0b), accV2);
-            accC = fma(qa.q, mcx_tile_newton(r2a, r0a), accC);
-            accC2 = fma(qb.q, mcx_tile_newton(r2b, r0b), accC2);
+        for (; j + K - 1 < j1; j += K) {
+            int4 p[K];
+            TileAtom q[K];
+#pragma unroll
+            for (int k = 0; k < K; k++) { p[k] = ipos[j + k]; q[k] = tq[j + k]; }
+            double r2[K], r0[K];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                r2[k] = mcx_tile_r2<PB>(s, pi, p[k], xi, yi, zi, j + k);
+                r0[k] = __builtin_amdgcn_rsq(r2[k]);
+            }
+#pragma unroll
+            for (int k = 0; k < K; k++)
+                if (scalar_nonzero(q[k].eps)) {
+                    double &av = (k & 1) ? accV2 : accV;
+                    av = fma((double)q[k].eps, (double)mcx_tile_lj(si + q[k].sig, r0[k]), av);
+                }
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                double &ac = (k & 1) ? accC2 : accC;
+                ac = fma(q[k].q, mcx_tile_newton(r2[k], r0[k]), ac);
+            }
         }
         accC += accC2;
         accV += accV2;
@@ -163,43 +181,61 @@ __global__ __launch_bounds__(kTile) void k_full_energy(DevSys s, const int2 *__r
 // ---------------------------------------------------------------------------
 
 constexpr int kSwWaves = 4;
-constexpr int kSwUnroll = 4;    // 64-entry slices of a row whose loads are in flight together
+constexpr int kSwUnroll = 4;    // entries per lane whose loads are in flight together
 
-// One wavefront per row.  The row's j > i suffix is read in chunks of
-// kSwUnroll x 64 entries: first every index of the chunk, then every j
-// atom's position and parameters, then the arithmetic, so that a chunk
-// waits on two memory latencies instead of two per 64 entries.  Each lane
-// keeps its own sums over all rows of the wavefront (q_i and eps_i folded
-// in per row); one wavefront sum at the end.
-template <int PB>
+// LPR lanes per row, 64/LPR consecutive rows per wavefront.  A row's j > i
+// suffix averages ~290 entries on water, so a wavefront per row leaves
+// ~40 % of its lanes idle in the last chunk; a group of LPR lanes per row
+// keeps them busy (consecutive rows -- the atoms of one or two molecules --
+// have suffixes of similar length).  Each lane walks its row with stride
+// LPR, kSwUnroll entries at a time: the j atoms' positions and parameters
+// are gathered for the step's indices, the next step's indices are loaded,
+// then the arithmetic runs while those are in flight.  A lane keeps its own sums
+// (q_i and eps_i folded in at the end of the row); one wavefront sum at the
+// end.
+template <int PB, int LPR>
 __global__ __launch_bounds__(64 * kSwWaves) void k_switch_energy(DevSys s, double *__restrict__ partial)
 {
+    constexpr int RPW = 64 / LPR;                // rows per wavefront
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
+    const int sub = lane % LPR;
     double ec = 0.0, ev = 0.0;
     // kernel length unit -> A^2
     const double u2 = PB == 1 ? s.unit[0] * s.unit[0] : 1.0;
-    for (int i = blockIdx.x * kSwWaves + w; i < s.n; i += gridDim.x * kSwWaves) {
-        const int4 pi = s.ipos[i];
-        const float2 ei = s.esk[i];
-        const double xi = PB == 0 ? s.x[i] : 0.0, yi = PB == 0 ? s.y[i] : 0.0, zi = PB == 0 ? s.z[i] : 0.0;
-        const int *__restrict__ row = s.nbr + (size_t)i * s.cap;
-        const int k1 = s.cnt[i];
+    for (int i0 = (blockIdx.x * kSwWaves + w) * RPW; i0 < s.n; i0 += gridDim.x * kSwWaves * RPW) {
+        const int i = i0 + lane / LPR;
+        const bool live = i < s.n;
+        const int ii = live ? i : i0;
+        const int4 pi = s.ipos[ii];
+        const float2 ei = s.esk[ii];
+        const double xi = PB == 0 ? s.x[ii] : 0.0, yi = PB == 0 ? s.y[ii] : 0.0, zi = PB == 0 ? s.z[ii] : 0.0;
+        const int *__restrict__ row = s.nbr + (size_t)ii * s.cap;
+        const int k1 = live ? s.cnt[ii] : 0;
         double c = 0.0, v = 0.0;
-        for (int kb = s.half[i]; kb < k1; kb += 64 * kSwUnroll) {
-            int j[kSwUnroll];
+        int kb = (live ? s.half[ii] : 0) + sub;
+        int j[kSwUnroll];
 #pragma unroll
-            for (int u = 0; u < kSwUnroll; u++) {
-                const int k = kb + 64 * u + lane;
-                j[u] = k < k1 ? row[k] : -1;
-            }
+        for (int u = 0; u < kSwUnroll; u++) {
+            const int k = kb + LPR * u;
+            j[u] = k < k1 ? row[k] : -1;
+        }
+        while (__ballot(kb < k1) != 0ull) {
             int4 pj[kSwUnroll];
             float2 ej[kSwUnroll];
 #pragma unroll
             for (int u = 0; u < kSwUnroll; u++) {
-                const int jj = j[u] < 0 ? i : j[u];    // a valid address; the pair is not summed
+                const int jj = j[u] < 0 ? ii : j[u];   // a valid address; the pair is not summed
                 pj[u] = s.ipos[jj];
                 ej[u] = s.esk[jj];
+            }
+            // the next step's indices, in flight during this step's arithmetic
+            int jn[kSwUnroll];
+            kb += LPR * kSwUnroll;
+#pragma unroll
+            for (int u = 0; u < kSwUnroll; u++) {
+                const int k = kb + LPR * u;
+                jn[u] = k < k1 ? row[k] : -1;
             }
 #pragma unroll
             for (int u = 0; u < kSwUnroll; u++) {
@@ -224,6 +260,8 @@ __global__ __launch_bounds__(64 * kSwWaves) void k_switch_energy(DevSys s, doubl
                     v = fma(sw, vj, v);
                 }
             }
+#pragma unroll
+            for (int u = 0; u < kSwUnroll; u++) j[u] = jn[u];
         }
         ec = fma((double)__int_as_float(pi.w), c, ec);
         ev = fma((double)ei.x, v, ev);
@@ -283,18 +321,47 @@ void launch_full_energy_listless(const DevSys &s, double *partial, hipStream_t s
         hipLaunchKernelGGL((k_full_energy<0, false>), dim3(ntiles), dim3(kTile), 0, st, s, nullptr, partial, nt);
 }
 
+// Lanes per row of the switched kernel: the fewest of 16, 32, 64 that still
+// give at least 24 wavefronts per CU (6,144), so that enough of them are in
+// flight to cover the load latency (MCX_SW_LPR = 8, 16, 32 or 64 forces one).
+static int sw_lpr(int n)
+{
+    static const int forced = [] {
+        const char *e = std::getenv("MCX_SW_LPR");
+        const int k = e ? std::atoi(e) : 0;
+        return (k == 8 || k == 16 || k == 32 || k == 64) ? k : 0;
+    }();
+    if (forced) return forced;
+    for (int k = 16; k < 64; k *= 2)
+        if ((long long)n * k / 64 >= 6144) return k;
+    return 64;
+}
+
 int switch_energy_blocks(int n)
 {
-    int b = (n + kSwWaves - 1) / kSwWaves;
+    const int rows = kSwWaves * (64 / sw_lpr(n));
+    int b = (n + rows - 1) / rows;
     return b < 1 ? 1 : b;
+}
+
+template <int LPR>
+static void launch_switch_lpr(const DevSys &s, double *partial, int nblocks, hipStream_t st)
+{
+    const int pb = s.pbtype == 0 ? 0 : (s.pbtype == 1 ? 1 : 2);
+    const dim3 grid(nblocks), block(64 * kSwWaves);
+    if (pb == 1) hipLaunchKernelGGL((k_switch_energy<1, LPR>), grid, block, 0, st, s, partial);
+    else if (pb == 2) hipLaunchKernelGGL((k_switch_energy<2, LPR>), grid, block, 0, st, s, partial);
+    else hipLaunchKernelGGL((k_switch_energy<0, LPR>), grid, block, 0, st, s, partial);
 }
 
 void launch_switch_energy(const DevSys &s, double *partial, int nblocks, hipStream_t st)
 {
-    const int pb = s.pbtype == 0 ? 0 : (s.pbtype == 1 ? 1 : 2);
-    if (pb == 1) hipLaunchKernelGGL(k_switch_energy<1>, dim3(nblocks), dim3(64 * kSwWaves), 0, st, s, partial);
-    else if (pb == 2) hipLaunchKernelGGL(k_switch_energy<2>, dim3(nblocks), dim3(64 * kSwWaves), 0, st, s, partial);
-    else hipLaunchKernelGGL(k_switch_energy<0>, dim3(nblocks), dim3(64 * kSwWaves), 0, st, s, partial);
+    switch (sw_lpr(s.n)) {
+    case 8: launch_switch_lpr<8>(s, partial, nblocks, st); break;
+    case 32: launch_switch_lpr<32>(s, partial, nblocks, st); break;
+    case 64: launch_switch_lpr<64>(s, partial, nblocks, st); break;
+    default: launch_switch_lpr<16>(s, partial, nblocks, st); break;
+    }
 }
 
 void launch_reduce(const double *partial, int nblocks, int ncomp, double *out, hipStream_t st)
