@@ -34,7 +34,9 @@
 // fourth row of the group.
 #include "listcore.h"
 
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #pragma clang fp contract(off)
 
@@ -512,6 +514,287 @@ void launch_groups(const DevSys &s, const CellGrid &g, int words, float cut2, fl
         hipLaunchKernelGGL((k_build_groups<G, false>), grid, block, lds, st, s, g, words, cut2, band, center_ok);
 }
 
+// ---------------------------------------------------------------------------
+// Rows of a group of up to kMaxGroup atoms of one cell, from the group's
+// candidates in ascending atom order (four wavefronts).
+//
+//  1. Every candidate record of the cells around the group's cell sets its
+//     atom's bit in one LDS bitmap over all atoms (one bitmap per group, not
+//     per row).
+//  2. The bitmap, scanned in word order, gives the candidates in ascending
+//     atom order, kCand at a time.
+//  3. Each candidate is tested against every row atom (the FP32 prefilter
+//     about the cell centre, the reference's FP64 test inside the band), a
+//     32-bit hit mask per candidate.
+//  4. Exclusions and the self pair clear their row's bit (a binary search in
+//     the sorted candidates).
+//  5. Each row takes its candidates in order: a ballot of the row's bit over
+//     64 candidates gives each hit its place, so the row is written in
+//     ascending order with coalesced stores; rows carry their counts from one
+//     batch of candidates to the next.
+constexpr int kMaxGroup = 32;      // rows of a group (hit masks are 32 bits)
+constexpr int kCand = 4096;        // sorted candidates handled per batch
+constexpr int kSRT = 256;          // threads of the sorted-candidate kernel
+constexpr int kSRW = kSRT / 64;
+
+size_t sorted_lds_bytes(int words) { return (size_t)words * 4; }
+
+template <int G, bool PBC>
+__global__ __launch_bounds__(kSRT) void k_build_sorted(DevSys s, CellGrid g, int words, float cut2, float band,
+                                                       int center_ok)
+{
+    constexpr int RW = G / kSRW;            // rows per wavefront in the ordered output
+    extern __shared__ uint32_t bits[];      // [words] the group's candidate atoms
+    __shared__ int cand[kCand];
+    __shared__ uint32_t hit[kCand];
+    __shared__ int run_start[kMaxRuns], run_len[kMaxRuns], run_pref[kMaxRuns + 1];
+    __shared__ int s_a[G], s_xpref[kMaxGroup + 1];
+    __shared__ int4 s_pi[G];                // row atoms' cell records (fixed-point position, 2*atom + frozen)
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int grp = blockIdx.x;
+    const int ngrp = g.group_start[g.ncell];
+    if (grp >= ngrp) return;                          // uniform over the block
+    const int4 gi = g.group_info[grp];                // (cell, first record, atoms)
+    const int na = min(gi.z, G), ci = gi.x;
+    for (int k = tid; k < words; k += kSRT) bits[k] = 0u;
+    if (tid < na) {
+        s_a[tid] = g.cell_atoms[gi.y + tid];
+        if (PBC) s_pi[tid] = g.cfix[gi.y + tid];
+    }
+    const int nrun = 2 * cell_runs(g, ci, tid, run_start, run_len);
+    __syncthreads();
+    if (tid < 64) {
+        const int incl = wave_incl_scan(tid < nrun ? run_len[tid] : 0);
+        run_pref[tid + 1] = incl;
+        if (tid == 0) run_pref[0] = 0;
+        // exclusion partners of each row (plus the self pair), as a prefix
+        const int nx = tid < na ? s.xfull_off[s_a[tid] + 1] - s.xfull_off[s_a[tid]] + 1 : 0;
+        const int xi = wave_incl_scan(nx);
+        if (tid < kMaxGroup) s_xpref[tid + 1] = xi;
+        if (tid == 0) s_xpref[0] = 0;
+    }
+    // row atoms in registers: FP32 offsets from the cell centre (periodic),
+    // fixed-point positions, frozen flags
+    int4 pc = make_int4(0, 0, 0, 0);
+    if (PBC) {
+        const int cc[3] = {ci % g.nc[0], (ci / g.nc[0]) % g.nc[1], ci / (g.nc[0] * g.nc[1])};
+        pc.x = (int)(unsigned)(long long)((cc[0] + 0.5) / g.nc[0] * 4294967296.0);
+        pc.y = (int)(unsigned)(long long)((cc[1] + 0.5) / g.nc[1] * 4294967296.0);
+        pc.z = (int)(unsigned)(long long)((cc[2] + 0.5) / g.nc[2] * 4294967296.0);
+    }
+    const float ux = (float)s.unit[0], uy = (float)s.unit[1], uz = (float)s.unit[2];
+    __syncthreads();
+    float ox[G], oy[G], oz[G];
+    unsigned fmask = 0u;
+#pragma unroll
+    for (int q = 0; q < G; q++) {
+        const bool in = q < na;
+        if (PBC) {
+            const int4 p = s_pi[in ? q : 0];
+            if (in && (p.w & 1)) fmask |= 1u << q;
+            ox[q] = in ? (float)wdiff(p.x, pc.x) * ux : 1e18f;
+            oy[q] = (float)wdiff(p.y, pc.y) * uy;
+            oz[q] = (float)wdiff(p.z, pc.z) * uz;
+        } else if (in && s.frozen[s_a[q]]) {
+            fmask |= 1u << q;
+        }
+    }
+    // 1. the candidate atoms' bits
+    {
+        constexpr int U = 8;                          // record loads in flight per thread
+        const int total = run_pref[nrun];
+        int r = 0;
+        for (int base = 0; base < total; base += kSRT * U) {
+            int j[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int f = base + u * kSRT + tid;
+                j[u] = -1;
+                if (f < total) {
+                    while (run_pref[r + 1] <= f) r++;
+                    j[u] = g.cell_atoms[run_start[r] + f - run_pref[r]];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (j[u] >= 0) atomicOr(&bits[j[u] >> 5], 1u << (j[u] & 31));
+        }
+    }
+    __syncthreads();
+    const unsigned live = na >= 32 ? ~0u : ((1u << na) - 1u);
+    const int nxt = s_xpref[na];
+    // this wavefront's rows of the ordered output: wv, wv + 4, ...
+    int r_cnt[RW], r_low[RW], r_a[RW];
+#pragma unroll
+    for (int t = 0; t < RW; t++) {
+        const int q = wv + kSRW * t;
+        r_cnt[t] = 0;
+        r_low[t] = 0;
+        r_a[t] = q < na ? s_a[q] : 0;
+    }
+
+    // 3-5 over the batch cand[0, nc)
+    auto process = [&](int nc) {
+        // 3. hit masks; the next candidate's position is loaded during this
+        // one's tests
+        int4 qn = make_int4(0, 0, 0, 0);
+        if (PBC && tid < nc) qn = g.ipos_ref[cand[tid]];
+        for (int k = tid; k < nc; k += kSRT) {
+            const int j = cand[k];
+            unsigned hits = 0u;
+            bool exact = !PBC;
+            if (PBC) {
+                const int4 q = qn;
+                if (k + kSRT < nc) qn = g.ipos_ref[cand[k + kSRT]];
+                float dmin = 3.0e38f;
+                if (center_ok) {
+                    const float cx = (float)wdiff(q.x, pc.x) * ux;
+                    const float cy = (float)wdiff(q.y, pc.y) * uy;
+                    const float cz = (float)wdiff(q.z, pc.z) * uz;
+#pragma unroll
+                    for (int r = 0; r < G; r++) {
+                        if (r >= na) break;
+                        const float dx = cx - ox[r], dy = cy - oy[r], dz = cz - oz[r];
+                        const float d = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) - cut2;
+                        dmin = fminf(dmin, fabsf(d));
+                        hits |= d < 0.f ? (1u << r) : 0u;
+                    }
+                } else {
+                    // small boxes: the row atoms' own minimum images (LDS)
+                    for (int r = 0; r < na; r++) {
+                        const int4 p = s_pi[r];
+                        const float dx = (float)wdiff(q.x, p.x) * ux;
+                        const float dy = (float)wdiff(q.y, p.y) * uy;
+                        const float dz = (float)wdiff(q.z, p.z) * uz;
+                        const float d = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) - cut2;
+                        dmin = fminf(dmin, fabsf(d));
+                        hits |= d < 0.f ? (1u << r) : 0u;
+                    }
+                }
+                exact = dmin <= band;
+            }
+            if (exact) {
+                // FP64 test of every row (the reference's r2 and test)
+                const double4 pjd = make_double4(s.x[j], s.y[j], s.z[j], 0.0);
+                hits = 0u;
+                for (int r = 0; r < na; r++) {
+                    const int aq = s_a[r];
+                    if (within<PBC>(s, pjd, s.x[aq], s.y[aq], s.z[aq])) hits |= 1u << r;
+                }
+            }
+            hits &= live;
+            if (fmask && s.frozen[j]) hits &= ~fmask;
+            hit[k] = hits;
+        }
+        __syncthreads();
+        // 4. exclusions and the self pair: clear the row's bit where the
+        // partner is among this batch's candidates
+        for (int t = tid; t < nxt; t += kSRT) {
+            int r = 0;
+            while (s_xpref[r + 1] <= t) r++;
+            const int aq = s_a[r];
+            const int x = t - s_xpref[r];
+            const int e = x == 0 ? aq : s.xfull_idx[s.xfull_off[aq] + x - 1];
+            int lo = 0, hi = nc;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (cand[mid] < e) lo = mid + 1; else hi = mid;
+            }
+            if (lo < nc && cand[lo] == e) atomicAnd(&hit[lo], ~(1u << r));
+        }
+        __syncthreads();
+        // 5. this wavefront's rows, in ascending order
+        for (int base = 0; base < nc; base += 64) {
+            const int k = base + lane;
+            const uint32_t m = k < nc ? hit[k] : 0u;
+            const int j = k < nc ? cand[k] : 0x7fffffff;
+#pragma unroll
+            for (int t = 0; t < RW; t++) {
+                const int q = wv + kSRW * t;
+                if (q >= na) break;
+                const bool h = (m >> q) & 1u;
+                const uint64_t b = __ballot(h);
+                if (h) {
+                    const int pos = r_cnt[t] + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32),
+                                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+                    if (pos < s.cap) s.nbr[(size_t)r_a[t] * s.cap + pos] = j;
+                }
+                r_low[t] += __popcll(b & __ballot(j < r_a[t]));
+                r_cnt[t] += __popcll(b);
+            }
+        }
+        __syncthreads();
+    };
+
+    // 2. candidates in ascending atom order, kCand at a time (a chunk of 64
+    // words holds at most 2,048 atoms, so it always fits after a flush);
+    // every wavefront scans the same chunk, lanes of wavefront w write the
+    // words k = w (mod 4)
+    int fill = 0;
+    for (int base = 0; base < words; base += 64) {
+        const int k = base + lane;
+        const uint32_t wd = k < words ? bits[k] : 0u;
+        const int pc_ = __popc(wd);
+        const int incl = wave_incl_scan(pc_);
+        const int tot = __builtin_amdgcn_readlane(incl, 63);
+        if (tot == 0) continue;                      // uniform: every wavefront reads the same words
+        if (fill + tot > kCand) {
+            __syncthreads();
+            process(fill);
+            fill = 0;
+        }
+        if ((lane & (kSRW - 1)) == wv) {
+            uint32_t w = wd;
+            int pos = fill + incl - pc_;
+            while (w) {
+                const int b = __ffs(w) - 1;
+                w &= w - 1u;
+                cand[pos++] = k * 32 + b;
+            }
+        }
+        fill += tot;
+    }
+    __syncthreads();
+    if (fill > 0) process(fill);
+    if (lane == 0) {
+#pragma unroll
+        for (int t = 0; t < RW; t++) {
+            const int q = wv + kSRW * t;
+            if (q >= na) break;
+            const int c = r_cnt[t], lo = r_low[t];
+            s.cnt[r_a[t]] = c < s.cap ? c : s.cap;
+            s.half[r_a[t]] = lo < s.cap ? lo : s.cap;
+            if (c > s.cap) atomicMax(s.overflow, c);
+        }
+    }
+}
+
+// Group size of the sorted-candidate kernel: 16 rows (the instance with 16
+// row registers; 8, 12, 24 and 32 measured slower on the 92,384-atom box),
+// or MCX_LIST_GROUP (1-32).
+int sorted_group_size()
+{
+    const char *e = std::getenv("MCX_LIST_GROUP");
+    const int v = e ? std::max(0, std::min(kMaxGroup, std::atoi(e))) : 0;
+    return v ? v : 16;
+}
+
+// the kernel instance (row registers) for a group size
+int sorted_template(int group) { return group <= 8 ? 8 : group <= 16 ? 16 : group <= 24 ? 24 : 32; }
+
+template <int G>
+void launch_sorted(const DevSys &s, const CellGrid &g, int group, int words, float cut2, float band, int center_ok,
+                   hipStream_t st)
+{
+    // groups: at most ceil(n / group) + ncell; surplus blocks return at once
+    const dim3 grid((s.n + group - 1) / group + g.ncell), block(kSRT);
+    const size_t lds = sorted_lds_bytes(words);
+    if (s.pbtype != 0)
+        hipLaunchKernelGGL((k_build_sorted<G, true>), grid, block, lds, st, s, g, words, cut2, band, center_ok);
+    else
+        hipLaunchKernelGGL((k_build_sorted<G, false>), grid, block, lds, st, s, g, words, cut2, band, center_ok);
+}
+
 }  // namespace
 
 size_t list_build_lds_bytes(int n)
@@ -524,8 +807,19 @@ void launch_list_build(const DevSys &s, const CellGrid &g, hipStream_t st)
 {
     const int words = (s.n + 31) / 32;
     const size_t per_row = (size_t)words * 4;
+    // Rows from sorted candidates from 32,768 atoms up, while the group's
+    // bitmap fits beside the candidate batch (up to ~229k atoms): there the
+    // per-row bitmaps of k_build_groups cost O(N) per row (92,384 atoms:
+    // 1.18 -> 0.80 ms).  Below, k_build_groups is faster (12,288 atoms:
+    // 126 us against 135 us at best), its rows' bitmaps being short.
+    // MCX_LIST_SORTED=1 (0) takes (never takes) the sorted kernel at any size.
+    const char *sv = std::getenv("MCX_LIST_SORTED");
+    const bool fits = sorted_lds_bytes(words) <= 28 * 1024;
+    const bool sorted = fits && (sv ? sv[0] == '1' : s.n >= 32768);
     int G = 8;
-    while (G > 1 && per_row * G > 60 * 1024) G >>= 1;
+    if (sorted) G = sorted_group_size();
+    else
+        while (G > 1 && per_row * G > 60 * 1024) G >>= 1;
     if (g.ncell <= kBinCells) {
         const int nb = (s.n + kBinAtoms - 1) / kBinAtoms;
         hipLaunchKernelGGL(k_bin_count, dim3(nb), dim3(kBinT), 0, st, s, g, G);
@@ -551,6 +845,15 @@ void launch_list_build(const DevSys &s, const CellGrid &g, hipStream_t st)
     for (int d = 0; d < 3; d++)
         if (!(0.5 * s.L[d] > rcut + 0.5 * std::sqrt(diag2) + 1e-3)) center_ok = 0;
     const float cut2 = (float)s.cutnb2, band = (float)(s.cutnb2 * kBand);
+    if (sorted) {
+        switch (sorted_template(G)) {
+        case 8: launch_sorted<8>(s, g, G, words, cut2, band, center_ok, st); break;
+        case 16: launch_sorted<16>(s, g, G, words, cut2, band, center_ok, st); break;
+        case 24: launch_sorted<24>(s, g, G, words, cut2, band, center_ok, st); break;
+        default: launch_sorted<32>(s, g, G, words, cut2, band, center_ok, st); break;
+        }
+        return;
+    }
     switch (G) {
     case 8: launch_groups<8>(s, g, words, cut2, band, center_ok, st); break;
     case 4: launch_groups<4>(s, g, words, cut2, band, center_ok, st); break;

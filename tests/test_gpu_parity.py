@@ -192,6 +192,46 @@ def test_list_pairs_at_the_list_radius(box):
     _check_list(ff.h, o)
 
 
+LIST_CASES = {
+    "argon864": lambda: systems.argon_fcc(seed=1)[0],
+    "water1536": lambda: systems.water_box(seed=3, nside=8, L=24.85)[0],
+    "water_small_box": lambda: systems.water_box(seed=21, nside=5, L=(90 / 0.0334) ** (1 / 3), nmol=90)[0],
+    "bonded_mix": lambda: systems.bonded_mix(seed=7)[0],
+    "bonded_mix_vacuum": lambda: systems.bonded_mix(seed=7, pbtype=systems.PBC_NONE)[0],
+    "argon_small_box": lambda: systems.argon_fcc(seed=9, ncell=3, L=17.2)[0],
+}
+
+
+@pytest.mark.parametrize("group", ["16", "5", "32"])
+@pytest.mark.parametrize("name", list(LIST_CASES))
+def test_sorted_candidate_list_build(name, group, monkeypatch):
+    """The sorted-candidate list kernel (taken by default from 32,768 atoms
+    up, here forced with MCX_LIST_SORTED=1 at every size and group size):
+    the group's candidates in ascending atom order from one bitmap, hit masks
+    per candidate, ordered row output.  The list must equal the oracle's bit
+    for bit: cubic, orthorhombic and vacuum boxes, boxes too small for the
+    cell-centre prefilter, frozen atoms, exclusions and 1-4 partners."""
+    monkeypatch.setenv("MCX_LIST_SORTED", "1")
+    monkeypatch.setenv("MCX_LIST_GROUP", group)
+    s = LIST_CASES[name]()
+    if name == "argon864":
+        s.frozen[: s.natom // 3] = 1
+    o, ref = _pair(s)
+    o.update_list()
+    ff = mcx.FField(s)
+    print("case", name, "group", group, flush=True)
+    _check_list(ff.h, o)
+    # after coordinates move: a rebuild through askListUpdate
+    rng = np.random.default_rng(5)
+    for arr in (s.x, s.y, s.z):
+        arr += rng.uniform(-0.3, 0.3, s.natom)
+    ref.x[:], ref.y[:], ref.z[:] = s.x, s.y, s.z
+    ff.set_coords(s.x, s.y, s.z)
+    ff.askListUpdate(50)
+    o.update_list()
+    _check_list(ff.h, o)
+
+
 def test_frozen_pairs_excluded():
     s, _ = systems.argon_fcc(seed=4)
     s.frozen[: s.natom // 2] = 1
