@@ -884,13 +884,18 @@ struct RingDraws {
 };
 
 // The draws of one proposal without its loads: exactly what pick_instance
-// and pick_displacement consume for a TRN or ROT move.
+// and pick_displacement consume for a TRN, ROT or force-bias move.
 template <class D>
 __device__ __forceinline__ void skip_proposal(const DevMoves &mv, D &dr, int lane)
 {
     const int tp = (int)((double)mv.ntypes * dr.next(lane));
     (void)dr.next(lane);
     const int type = mv.type[tp < mv.ntypes ? tp : mv.ntypes - 1];
+    if (type == MCX_MV_FBIAS_DEV) {
+        // force bias: four draws of the Box-Muller normals
+        for (int k = 0; k < 4; k++) (void)dr.next(lane);
+        return;
+    }
     double r0, r1, r2;
     do {
         r0 = dr.next(lane) * 2.0 + (-1.0);
@@ -912,6 +917,7 @@ struct WideLDS {
     double accu[kMaxWide];             // its acceptance draw
     int nprop;                         // proposals of the round (0: the chunk is over)
     int fail;
+    double lnq;                        // force bias: log proposal-density ratio of this workgroup's proposal
 };
 
 // Grid-wide barrier of a round: every workgroup's writes before it are
@@ -1178,7 +1184,59 @@ __global__ __launch_bounds__(kMCT) void k_mc_wide(DevSys s0, DevTerms t, DevMove
         // sums go to the board, where the resolution adds them in the
         // order of delta_finish; meanwhile wavefront 0 fills the draw ring
         // and parses ahead for the next round
-        if (prop < np) {
+        if (prop < np && m.type == MCX_MV_FBIAS_DEV) {
+            // force-bias proposal (one workgroup per proposal): the net force
+            // on the set at its old position, the drifted trial, then dE and
+            // the force at the new position, reduced as the one-proposal
+            // kernel reduces them; dE goes to the board as the bonded part of
+            // wavefront 0 (the other sums zero, so the resolution's sum is
+            // dE exactly) and the log proposal-density ratio beside it
+            force_old(s, m);
+            if (w0) {
+                if (lane == 0) {
+                    const double A = dmax[m.acc], bA = arg.beta * A * mv.fb_scale[m.acc], sq = sqrt(2.0 * A);
+                    for (int c = 0; c < 3; c++) {
+                        m.xi[c] = m.r[c];
+                        m.r[c] = bA * m.fold[c] + sq * m.xi[c];
+                    }
+                }
+                wave_sync();
+                trial_w0(s, m, lane, arg.ipos_ref, arg.half_skin2);
+                if (m.scan) prefix_w0(s, m, lane);
+            }
+            __syncthreads();
+            double dec = 0.0, dev = 0.0, deb = 0.0;
+            double fnew[3] = {0.0, 0.0, 0.0};
+            if (w0) delta_terms(s, mv, m, dec, dev, deb);
+            else delta_pairs(s, m, true, dec, dev, fnew);
+            delta_finish(m, true, dec, dev, deb, fnew);
+            if (w0) {
+                if (lane == 0) {
+                    const int tp = m.acc;
+                    const double A = dmax[tp], bA = arg.beta * A * mv.fb_scale[tp];
+                    double fw = 0.0, rv = 0.0;
+                    for (int c = 0; c < 3; c++) {
+                        const double a = m.r[c] - bA * m.fold[c], b = -m.r[c] - bA * m.fnew[c];
+                        fw += a * a;
+                        rv += b * b;
+                    }
+                    g.lnq = (fw - rv) / (4.0 * A);
+                    w.blnq[prop] = g.lnq;
+                }
+                if (lane < kMCW) {
+                    double *r = w.bred + ((size_t)prop * kMCW + lane) * 3;
+                    r[0] = 0.0; r[1] = 0.0; r[2] = lane == 0 ? m.de : 0.0;
+                }
+                if (lane == 0) w.bmeta[prop] = make_int4(m.acc, m.first, m.nm, m.inst);
+                for (int a = lane; a < m.nm; a += 64) {
+                    double *b = w.bxyz + ((size_t)prop * kMaxMoveAtoms + a) * 3;
+                    b[0] = m.tx[a]; b[1] = m.ty[a]; b[2] = m.tz[a];
+                    w.bip[(size_t)prop * kMaxMoveAtoms + a] =
+                        mcx_fixed4(s, m.tx[a], m.ty[a], m.tz[a], __int_as_float(m.ip[a].w));
+                }
+                parse_ahead();
+            }
+        } else if (prop < np) {
             double dec = 0.0, dev = 0.0, deb = 0.0;
             double fnew[3] = {0.0, 0.0, 0.0};
             const int v0 = 1 + part * (kMCW - 1) / w.split, v1 = 1 + (part + 1) * (kMCW - 1) / w.split;
@@ -1238,8 +1296,13 @@ __global__ __launch_bounds__(kMCT) void k_mc_wide(DevSys s0, DevTerms t, DevMove
                 cip = w.bip[(size_t)pq * kMaxMoveAtoms + pa];
             }
             // MC_metropolis::apply_criterion (src/MC_metropolis.cpp:175-204)
-            const bool free_acc = de_l <= DBL_EPSILON;
-            const bool acc_l = lane < np && (free_acc || g.accu[lane < kMaxWide ? lane : 0] < exp(-arg.beta * de_l));
+            // force bias: min(1, exp(lnq - beta dE)), the acceptance draw taken
+            // only below 1 (as the one-proposal kernel tests it)
+            const bool fb_l = lane < np && mv.type[meta_l.x] == MCX_MV_FBIAS_DEV;
+            const double xfb = fb_l ? w.blnq[lane] - arg.beta * de_l : 0.0;
+            const bool free_acc = fb_l ? xfb >= 0.0 : de_l <= DBL_EPSILON;
+            const bool acc_l = lane < np && (free_acc || g.accu[lane < kMaxWide ? lane : 0] <
+                                                              (fb_l ? exp(xfb) : exp(-arg.beta * de_l)));
             const uint64_t accm = __ballot(acc_l);
             const int acc_at = accm ? __builtin_ctzll(accm) : -1;
             const int R = acc_at >= 0 ? acc_at + 1 : np;
@@ -1274,7 +1337,7 @@ __global__ __launch_bounds__(kMCT) void k_mc_wide(DevSys s0, DevTerms t, DevMove
                 if (arg.de_out) arg.de_out[o] = de_own;
                 if (arg.mvtyp_out) arg.mvtyp_out[o] = m.acc;
                 if (arg.mvinst_out) arg.mvinst_out[o] = m.inst;
-                if (arg.lnq_out) arg.lnq_out[o] = 0.0;
+                if (arg.lnq_out) arg.lnq_out[o] = m.type == MCX_MV_FBIAS_DEV ? g.lnq : 0.0;
                 if (arg.disp_out) {
                     const bool tr = m.type != MCX_MV_ROT_DEV;
                     for (int c = 0; c < 3; c++) arg.disp_out[3 * o + c] = tr ? m.r[c] : 0.0;

@@ -1043,15 +1043,18 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
         a.stream = h->rng_stream;
         a.beta = 1.0 / ((MCX_RBOLTZUI / MCX_KCALTOIU) * h->temp);
         // speculative rounds (several proposals evaluated together; the same
-        // chain) unless a force-bias type is present or MCX_SPECULATE=0
+        // chain) unless MCX_SPECULATE=0; move sets with force-bias types run
+        // them on the wide kernel only (one workgroup per proposal: its force
+        // pass precedes its trial), else one proposal at a time
         const char *sp = std::getenv("MCX_SPECULATE");
         a.spec = !(sp && sp[0] == '0');
         if (use_patch) {
             a.patch = h->patch();
             a.record_moves = 1;
         }
+        bool has_fb = false;
         for (int k = 0; k < ntypes; k++)
-            if (types[k].type == MCX_MV_FBIAS || types[k].type == MCX_MV_FBIAS_ACC) a.spec = 0;
+            if (types[k].type == MCX_MV_FBIAS || types[k].type == MCX_MV_FBIAS_ACC) has_fb = true;
         // speculative rounds over several workgroups (k_mc_wide) for the
         // same move sets; MCX_WIDE=<n> sets the workgroup count, 0 or 1
         // keeps the single-workgroup read-ahead kernel
@@ -1062,17 +1065,18 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
             const char *wv = std::getenv("MCX_WIDE");
             const char *sv = std::getenv("MCX_WIDE_SPLIT");
             const int want = wv ? std::atoi(wv) : kDefaultWide;
-            split = std::max(1, std::min(sv ? std::atoi(sv) : kDefaultSplit, 15));
+            split = has_fb ? 1 : std::max(1, std::min(sv ? std::atoi(sv) : kDefaultSplit, 15));
             const int cap = std::min(kMaxWide, mc_wide_max_workgroups(h->device)) / split;
             const int W = std::min(want, cap);
             if (want > 1 && W >= 2) nwg = W * split;
         }
+        if (has_fb && !nwg) a.spec = 0;
         DBuf<unsigned> sync_d, exit_d, arrive_d;
         DBuf<int> werr_d, go_d;
-        DBuf<double> bred_d, bxyz_d;
+        DBuf<double> bred_d, bxyz_d, blnq_d;
         DBuf<int4> bmeta_d, bip_d;
         cleanup.f.push_back([&] { sync_d.release(); exit_d.release(); arrive_d.release(); go_d.release();
-                                  werr_d.release(); bred_d.release(); bxyz_d.release();
+                                  werr_d.release(); bred_d.release(); bxyz_d.release(); blnq_d.release();
                                   bmeta_d.release(); bip_d.release(); });
         WideArgs wa{};
         if (nwg) {
@@ -1087,6 +1091,7 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
             bmeta_d.alloc(kMaxWide);
             bxyz_d.alloc((size_t)kMaxWide * kMaxMoveAtoms * 3);
             bip_d.alloc((size_t)kMaxWide * kMaxMoveAtoms);
+            blnq_d.alloc(kMaxWide);
             int rate_khz = 0;
             HIPCHK(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, h->device));
             wa.nwg = nwg;
@@ -1098,7 +1103,7 @@ int mcx_mc_run_traced(mcx_ctx *h, const mcx_move_type *types, int ntypes, int ns
             wa.arrive = arrive_d.p;
             wa.go = go_d.p;
             wa.split = split;
-            wa.bred = bred_d.p; wa.bmeta = bmeta_d.p; wa.bxyz = bxyz_d.p; wa.bip = bip_d.p;
+            wa.bred = bred_d.p; wa.bmeta = bmeta_d.p; wa.bxyz = bxyz_d.p; wa.bip = bip_d.p; wa.blnq = blnq_d.p;
         }
         // MCX_KERNEL_EVENTS=k (k >= 1): every k-th chunk and every k-th
         // in-run list update between a pair of HIP events (pairs from the

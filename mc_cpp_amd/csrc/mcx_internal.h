@@ -270,6 +270,7 @@ struct WideArgs {
     int4 *bmeta;             // [kMaxWide] (type index, first atom, moved atoms, instance)
     double *bxyz;            // [kMaxWide * kMaxMoveAtoms * 3] trial coordinates
     int4 *bip;               // [kMaxWide * kMaxMoveAtoms] trial fixed-point mirror
+    double *blnq;            // [kMaxWide] force bias: log proposal-density ratio of each proposal
 };
 // Launch of nwg workgroups.  The kernel starts with a rendezvous: unless
 // every workgroup is resident within rdv_ticks, it changes nothing, sets

@@ -162,6 +162,59 @@ def test_force_bias_mixed_with_reference_moves():
     assert abs(res["etot"] - e_fresh) <= 1e-6 * abs(e_fresh)
 
 
+FB_KERNELS = {"one_at_a_time": {"MCX_SPECULATE": "0"},
+              "wide_default": {},
+              "wide5": {"MCX_WIDE": "5"},
+              "wide16": {"MCX_WIDE": "16", "MCX_WIDE_SPLIT": "3"}}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["fbias", "fbias_acc_adapt", "mixed_philox"])
+def test_force_bias_wide_kernel_same_chain(case, monkeypatch):
+    """Move sets with force-bias types run speculative rounds on the wide
+    kernel: one workgroup per proposal computes the net force on its set,
+    the drifted trial, dE and the force at the new position (reduced as the
+    one-proposal kernel reduces them), and the rounds resolve in chain order
+    with the force-bias acceptance (the acceptance draw taken only for a
+    negative exponent).  Every configuration gives the chain of the
+    one-proposal kernel (MCX_SPECULATE=0) bit for bit: decisions, dE, ln q,
+    displacements, dmax, running total and coordinates (MCX_WIDE_SPLIT is
+    ignored: one workgroup per force-bias proposal)."""
+    import copy
+    s = _box(seed=8)
+    if case == "fbias":
+        moves, mode = [_fb_move(s, 1e-3)], mcx.MC_REPLAY
+    elif case == "fbias_acc_adapt":
+        m = _fb_move(s, 5e-6, systems.MV_FBIAS_ACC)
+        m.each = 70
+        moves, mode = [m], mcx.MC_REPLAY
+    else:
+        moves, mode = [s.moves[0], s.moves[1], _fb_move(s, 1e-3)], mcx.MC_PHILOX
+    n = 2000
+    out = {}
+    for name, env in FB_KERNELS.items():
+        for k in ("MCX_SPECULATE", "MCX_WIDE", "MCX_WIDE_SPLIT"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        sc = copy.deepcopy(s)
+        ff = mcx.FField(sc)
+        print("case", case, "kernel", name, flush=True)
+        res = ff.h.mc_run(copy.deepcopy(moves), n, 31, rng_mode=mode, list_every=50)
+        out[name] = (res, ff.get_coords())
+    r0, c0 = out["one_at_a_time"]
+    assert 0 < int(r0["acc"].sum()) < n
+    for name in FB_KERNELS:
+        r1, c1 = out[name]
+        for key in ("acc", "de", "lnq", "mvtyp", "mvinst", "disp", "trials", "accepted", "dmax"):
+            assert np.array_equal(r0[key], r1[key]), (name, key)
+        assert r0["etot"] == r1["etot"], name
+        for a, b in zip(c0, c1):
+            assert np.array_equal(a, b), name
+    if case == "fbias_acc_adapt":
+        assert r0["dmax"][0] != moves[0].dmax
+
+
 @pytest.mark.gpu
 def test_batch_refuses_force_bias():
     s = _box(seed=7)
