@@ -235,6 +235,16 @@ def kernel_probe(args):
     hf = mcx.from_system(sf)
     hf.energy()
     hf.time_kernel(0, 20)
+    large = ""
+    if args.probe_large:
+        # the 92,384-atom mixed box: its switched energy (16 lanes per row)
+        # and the sorted-candidate list build
+        sl, _ = systems.mixed_system(seed=3)
+        hl = mcx.from_system(sl)
+        hl.update_list()
+        hl.time_kernel(1, 20)
+        hl.time_kernel(2, 5)
+        large = ", mixed_92384: k_switch_energy x21, list build x6"
     src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
     torch.cuda.synchronize()
@@ -242,7 +252,7 @@ def kernel_probe(args):
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"probe": "mc_run %d+500 moves, k_switch_energy x21, list build x6, k_full_energy x21 "
-                               "(1 warm-up each), copy x5" % args.probe_moves,
+                               "(1 warm-up each)%s, copy x5" % (args.probe_moves, large),
                       "copy_bytes": COPY_BYTES, "atoms": s.natom}), flush=True)
 
 
@@ -368,6 +378,7 @@ def main():
     ap.add_argument("--no-fbias", action="store_true")
     ap.add_argument("--probe", action="store_true", help="kernel sequence for the HBM counter runs only")
     ap.add_argument("--probe-moves", type=int, default=2000)
+    ap.add_argument("--probe-large", action="store_true", help="the probe also runs the ~100k-atom system's kernels")
     ap.add_argument("--large-moves", type=int, default=10000, help="moves per run on the ~100k-atom system")
     ap.add_argument("--no-large", action="store_true")
     args = ap.parse_args()

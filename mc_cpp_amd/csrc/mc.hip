@@ -979,7 +979,9 @@ __device__ __forceinline__ long long next_adapt(const DevMoves &mv, long long st
     return best;
 }
 
-template <int PB, int CUT>
+// FB: the move set has force-bias types (their proposals carry a force pass
+// and the force-bias acceptance); without, the instance has no such code.
+template <int PB, int CUT, bool FB = false>
 __global__ __launch_bounds__(kMCT) void k_mc_wide(DevSys s0, DevTerms t, DevMoves mv, MCRunArgs arg, WideArgs w)
 {
     const DevSys s = specialise<PB, CUT>(s0);
@@ -1184,7 +1186,7 @@ __global__ __launch_bounds__(kMCT) void k_mc_wide(DevSys s0, DevTerms t, DevMove
         // sums go to the board, where the resolution adds them in the
         // order of delta_finish; meanwhile wavefront 0 fills the draw ring
         // and parses ahead for the next round
-        if (prop < np && m.type == MCX_MV_FBIAS_DEV) {
+        if (FB && prop < np && m.type == MCX_MV_FBIAS_DEV) {
             // force-bias proposal (one workgroup per proposal): the net force
             // on the set at its old position, the drifted trial, then dE and
             // the force at the new position, reduced as the one-proposal
@@ -1298,7 +1300,7 @@ __global__ __launch_bounds__(kMCT) void k_mc_wide(DevSys s0, DevTerms t, DevMove
             // MC_metropolis::apply_criterion (src/MC_metropolis.cpp:175-204)
             // force bias: min(1, exp(lnq - beta dE)), the acceptance draw taken
             // only below 1 (as the one-proposal kernel tests it)
-            const bool fb_l = lane < np && mv.type[meta_l.x] == MCX_MV_FBIAS_DEV;
+            const bool fb_l = FB && lane < np && mv.type[meta_l.x] == MCX_MV_FBIAS_DEV;
             const double xfb = fb_l ? w.blnq[lane] - arg.beta * de_l : 0.0;
             const bool free_acc = fb_l ? xfb >= 0.0 : de_l <= DBL_EPSILON;
             const bool acc_l = lane < np && (free_acc || g.accu[lane < kMaxWide ? lane : 0] <
@@ -1337,7 +1339,7 @@ __global__ __launch_bounds__(kMCT) void k_mc_wide(DevSys s0, DevTerms t, DevMove
                 if (arg.de_out) arg.de_out[o] = de_own;
                 if (arg.mvtyp_out) arg.mvtyp_out[o] = m.acc;
                 if (arg.mvinst_out) arg.mvinst_out[o] = m.inst;
-                if (arg.lnq_out) arg.lnq_out[o] = m.type == MCX_MV_FBIAS_DEV ? g.lnq : 0.0;
+                if (arg.lnq_out) arg.lnq_out[o] = FB && m.type == MCX_MV_FBIAS_DEV ? g.lnq : 0.0;
                 if (arg.disp_out) {
                     const bool tr = m.type != MCX_MV_ROT_DEV;
                     for (int c = 0; c < 3; c++) arg.disp_out[3 * o + c] = tr ? m.r[c] : 0.0;
@@ -1500,7 +1502,14 @@ hipError_t launch_mc_wide(const DevSys &s, const DevTerms &t, const DevMoves &mv
     DevMoves mv_ = mv;
     MCRunArgs a_ = a;
     WideArgs w_ = w;
-    if (s.pbtype == MCX_PBC_CUBIC && s.cutmode == MCX_CUT_SWITCH)
+    bool fb = false;
+    for (int k = 0; k < mv.ntypes; k++) fb = fb || mv.type[k] == MCX_MV_FBIAS_DEV;
+    if (fb && s.pbtype == MCX_PBC_CUBIC && s.cutmode == MCX_CUT_SWITCH)
+        hipLaunchKernelGGL((k_mc_wide<MCX_PBC_CUBIC, MCX_CUT_SWITCH, true>), dim3(w.nwg), dim3(kMCT), 0, st, s_, t_, mv_,
+                           a_, w_);
+    else if (fb)
+        hipLaunchKernelGGL((k_mc_wide<-1, -1, true>), dim3(w.nwg), dim3(kMCT), 0, st, s_, t_, mv_, a_, w_);
+    else if (s.pbtype == MCX_PBC_CUBIC && s.cutmode == MCX_CUT_SWITCH)
         hipLaunchKernelGGL((k_mc_wide<MCX_PBC_CUBIC, MCX_CUT_SWITCH>), dim3(w.nwg), dim3(kMCT), 0, st, s_, t_, mv_, a_, w_);
     else if (s.pbtype == MCX_PBC_CUBIC && s.cutmode == MCX_CUT_FULL)
         hipLaunchKernelGGL((k_mc_wide<MCX_PBC_CUBIC, MCX_CUT_FULL>), dim3(w.nwg), dim3(kMCT), 0, st, s_, t_, mv_, a_, w_);
