@@ -1134,6 +1134,18 @@ __global__ __launch_bounds__(kMCT) void k_mc_wide(DevSys s0, DevTerms t, DevMove
                 if (na != LLONG_MAX) wmax = min(wmax, na - ((long long)arg.step0 + done));
                 long long p = pos;
                 while (np < wmax && p + 64 - pos <= kWideRing) {
+                    // proposals already parsed: a compact loop (the parse and
+                    // generation below are rare, and keeping them out of this
+                    // loop keeps its code short)
+                    while (p < eq_hi) {
+                        const int e = g.endq[p & (kWideRing - 1)];
+                        if (e < 0) break;
+                        if (lane == np) g_start_l = p;
+                        p += e + 1;
+                        np++;
+                        if (!(np < wmax && p + 64 - pos <= kWideRing)) break;
+                    }
+                    if (!(np < wmax && p + 64 - pos <= kWideRing)) break;
                     if (p >= eq_hi) {
                         top_up(eq_hi + 192);
                         parse_chunk(eq_hi);
