@@ -594,7 +594,8 @@ def test_gradient_vs_fp64_and_fd():
         assert np.max(np.abs(g_ - o_)) <= 1e-4 * max(np.max(np.abs(o_)), 1e-12)
 
 
-@pytest.mark.parametrize("case", ["water1536", "bonded_mix", "water_small_box", "mixed_small_frozen", "water12288"])
+@pytest.mark.parametrize("case", ["water1536", "bonded_mix", "water_small_box", "mixed_small_frozen", "water12288",
+                                  "water12288_every2500"])
 def test_incremental_list_equals_full_rebuild(case, monkeypatch):
     """Scheduled rebuilds inside mcx_mc_run are incremental updates (rows of
     the atoms moved since the last update rebuilt, the other rows patched
@@ -603,7 +604,10 @@ def test_incremental_list_equals_full_rebuild(case, monkeypatch):
     oracle's list at the final coordinates (bit-exact: the same pairs in the
     same order).  water_small_box has a single cell; mixed_small_frozen has
     frozen atoms; water12288 runs long enough for more than 2,048 atoms to
-    move since the binning, which asks for a full build in between."""
+    move since the binning, which asks for a full build in between;
+    water12288_every2500 updates every 2,500 moves, so that one update takes
+    more than 1,024 moved atoms (more than one per thread of the row kernel,
+    more than one per block of its grid) and as many stale ones."""
     import copy
     if case == "water1536":
         s = systems.water_box(seed=3, nside=8, L=24.85)[0]
@@ -616,14 +620,15 @@ def test_incremental_list_equals_full_rebuild(case, monkeypatch):
         s.frozen[::7] = 1
     else:
         s = systems.water_box(seed=2)[0]
-    n = 12000 if case == "water12288" else 3000
+    n = {"water12288": 12000, "water12288_every2500": 5000}.get(case, 3000)
+    every = 2500 if case == "water12288_every2500" else 50
     out = {}
     for patch in ("0", "1"):
         monkeypatch.setenv("MCX_LIST_PATCH", patch)
         sc = copy.deepcopy(s)
         ff = mcx.FField(sc)
         print("case", case, "patch", patch, flush=True)
-        res = ff.h.mc_run(sc.moves, n, 4242, list_every=50)
+        res = ff.h.mc_run(sc.moves, n, 4242, list_every=every)
         out[patch] = (res, ff.get_coords(), ff.h.get_list())
     (r0, c0, l0), (r1, c1, l1) = out["0"], out["1"]
     assert 0 < int(r0["acc"].sum()) < n
@@ -633,7 +638,7 @@ def test_incremental_list_equals_full_rebuild(case, monkeypatch):
     for a, b in zip(c0, c1):
         assert np.array_equal(a, b)
     assert np.array_equal(l0[0], l1[0]) and np.array_equal(l0[1], l1[1])
-    # n is a multiple of 50: the list was brought up to date at the end
+    # n is a multiple of the update interval: the list was brought up to date at the end
     ref = s.copy()
     ref.x[:], ref.y[:], ref.z[:] = c1
     o = Oracle(ref)
