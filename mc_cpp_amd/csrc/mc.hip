@@ -1506,6 +1506,20 @@ __global__ void k_mirror(DevSys s)
     }
 }
 
+// The same from a packed upload (x[n], y[n], z[n]), which also becomes the
+// FP64 master copy.
+__global__ void k_mirror_packed(DevSys s, const double *__restrict__ xyz)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
+        const double x = xyz[i], y = xyz[s.n + i], z = xyz[2 * s.n + i];
+        s.x[i] = x;
+        s.y[i] = y;
+        s.z[i] = z;
+        const int qb = s.ipos[i].w;
+        s.ipos[i] = mcx_fixed4(s, x, y, z, __int_as_float(qb));
+    }
+}
+
 hipError_t launch_mc_wide(const DevSys &s, const DevTerms &t, const DevMoves &mv, const MCRunArgs &a,
                           const WideArgs &w, hipStream_t st)
 {
@@ -1564,4 +1578,9 @@ void launch_mc_chunk(const DevSys &s, const DevTerms &t, const DevMoves &mv, con
 void launch_mirror_from_coords(const DevSys &s, hipStream_t st)
 {
     hipLaunchKernelGGL(k_mirror, dim3((s.n + 255) / 256), dim3(256), 0, st, s);
+}
+
+void launch_mirror_from_packed(const DevSys &s, const double *xyz, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mirror_packed, dim3((s.n + 255) / 256), dim3(256), 0, st, s, xyz);
 }

@@ -370,22 +370,25 @@ void energy_device(mcx_ctx *h, double out9[9])
     if (h->cutmode == MCX_CUT_FULL) {
         if (h->ntiles == 0) setup_tiles(h);
         nb_nb = h->ntiles;
-        h->partial.alloc((size_t)std::max(nb_nb * 2, bonded_blocks(t) * 7));
+        h->partial.alloc((size_t)nb_nb * 2 + (size_t)bonded_blocks(t) * 7);
         launch_full_energy(s, h->tiles.p, h->ntiles, kFullTile, h->partial.p, h->stream);
     } else {
         if (!h->list_valid) build_list(h);
         s = h->sys();
         nb_nb = switch_energy_blocks(h->n);
-        h->partial.alloc((size_t)std::max(nb_nb * 2, bonded_blocks(t) * 7));
+        h->partial.alloc((size_t)nb_nb * 2 + (size_t)bonded_blocks(t) * 7);
         launch_switch_energy(s, h->partial.p, nb_nb, h->stream);
     }
-    launch_reduce(h->partial.p, nb_nb, 2, h->result.p, h->stream);
+    // the bonded partials after the nonbonded ones, both sums in one launch
     const int nbb = bonded_blocks(t);
-    launch_bonded(s, t, h->partial.p, nbb, h->stream);
-    launch_reduce(h->partial.p, nbb, 7, h->result.p + 2, h->stream);
+    double *pb = h->partial.p + (size_t)nb_nb * 2;
+    launch_bonded(s, t, pb, nbb, h->stream);
+    launch_reduce2(h->partial.p, nb_nb, 2, h->result.p, pb, nbb, 7, h->result.p + 2, h->stream);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(out9, h->result.p, 9 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (!h->res_h) HIPCHK(hipHostMalloc((void **)&h->res_h, 16 * sizeof(double), hipHostMallocDefault));
+    HIPCHK(hipMemcpyAsync(h->res_h, h->result.p, 9 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
+    std::memcpy(out9, h->res_h, 9 * sizeof(double));
 }
 
 void assemble(const double r[9], double ener[10])
@@ -862,14 +865,27 @@ int mcx_set_coords_moved(mcx_ctx *h, const double *x, const double *y, const dou
             upload_coords(h, x, y, z);
             return;
         }
-        const int n = h->n;
-        h->x.upload(x, n, h->stream);
-        h->y.upload(y, n, h->stream);
-        h->z.upload(z, n, h->stream);
-        h->hx.assign(x, x + n); h->hy.assign(y, y + n); h->hz.assign(z, z + n);
-        h->host_stale = false;
-        launch_mirror_from_coords(h->sys(), h->stream);
+        // one upload of x, y, z packed in pinned memory; the device copy is
+        // the master from here on (the host copy is read back when needed)
+        const size_t n = (size_t)h->n;
+        if (h->stage_ev) HIPCHK(hipEventSynchronize(h->stage_ev));    // the last upload from stage_h is done
+        else HIPCHK(hipEventCreateWithFlags(&h->stage_ev, hipEventDisableTiming));
+        if (h->stage_cap < 3 * n) {
+            if (h->stage_h) HIPCHK(hipHostFree(h->stage_h));
+            h->stage_h = nullptr;
+            h->stage_cap = 0;
+            HIPCHK(hipHostMalloc((void **)&h->stage_h, 3 * n * sizeof(double), hipHostMallocDefault));
+            h->stage_cap = 3 * n;
+        }
+        std::memcpy(h->stage_h, x, n * sizeof(double));
+        std::memcpy(h->stage_h + n, y, n * sizeof(double));
+        std::memcpy(h->stage_h + 2 * n, z, n * sizeof(double));
+        h->xyz_in.alloc(3 * n);
+        HIPCHK(hipMemcpyAsync(h->xyz_in.p, h->stage_h, 3 * n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipEventRecord(h->stage_ev, h->stream));
+        launch_mirror_from_packed(h->sys(), h->xyz_in.p, h->stream);
         HIPCHK(hipGetLastError());
+        h->host_stale = true;
         // the moves since the last list update were not recorded on the device
         h->patch_valid = false;
     });

@@ -73,6 +73,13 @@ struct mcx_ctx {
     DBuf<TileAtom> tq;
     DBuf<double4> qes;
     std::vector<double> hx, hy, hz;   // host copy of the last uploaded coordinates
+    // pinned host staging: mcx_set_coords_moved packs x, y, z into stage_h
+    // for one upload (stage_ev marks its end), mcx_energy's nine sums land
+    // in res_h
+    double *stage_h = nullptr, *res_h = nullptr;
+    size_t stage_cap = 0;
+    hipEvent_t stage_ev = nullptr;
+    DBuf<double> xyz_in;
     bool host_stale = false;          // the device coordinates were changed in place since
     DBuf<int> frozen_d, excl_off_d, excl_idx_d, xfull_off_d, xfull_idx_d, p14_off_d, p14_idx_d;
     DBuf<double4> cpos;
@@ -177,8 +184,12 @@ struct mcx_ctx {
     ~mcx_ctx()
     {
         DBuf<double> *d[] = {&x, &y, &z, &q64, &eps64, &sig64, &eps1464, &sig1464, &mass64, &partial, &result,
-                             &newxyz, &scalar};
+                             &newxyz, &scalar, &xyz_in};
         for (auto *b : d) b->release();
+        if (stage_ev) (void)hipEventSynchronize(stage_ev);
+        if (stage_h) (void)hipHostFree(stage_h);
+        if (res_h) (void)hipHostFree(res_h);
+        if (stage_ev) (void)hipEventDestroy(stage_ev);
         ipos.release(); es.release(); esk.release(); tq.release(); es14.release(); qes.release();
         cpos.release();
         cfix.release();

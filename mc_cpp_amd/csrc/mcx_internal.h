@@ -163,6 +163,9 @@ void launch_switch_energy(const DevSys &s, double *partial, int nblocks, hipStre
 // tile list: ceil(n/64)*(ceil(n/64)+1)/2 blocks, two partials each.
 void launch_full_energy_listless(const DevSys &s, double *partial, hipStream_t st);
 void launch_reduce(const double *partial, int nblocks, int ncomp, double *out, hipStream_t st);
+// the two sums of launch_reduce (pa -> oa, pb -> ob) in one launch
+void launch_reduce2(const double *pa, int na, int ca, double *oa, const double *pb, int nb, int cb, double *ob,
+                    hipStream_t st);
 int switch_energy_blocks(int n);
 // list.hip
 struct CellGrid {
@@ -289,6 +292,9 @@ void launch_type_sums(const double *g8, int n, const int *type, int ntypes, doub
 // mc.hip: the fixed-point mirror recomputed from the device coordinates
 // (charge bits kept)
 void launch_mirror_from_coords(const DevSys &s, hipStream_t st);
+// the FP64 coordinates from a packed device copy (x[n], y[n], z[n]) into
+// s.x, s.y, s.z, and the mirror from them
+void launch_mirror_from_packed(const DevSys &s, const double *xyz, hipStream_t st);
 // batch.hip: one Metropolis chain per replica, one workgroup each
 constexpr int kBatchTypes = 64;     // distinct (q, eps, sig) atom types of a replica
 constexpr int kBatchAtoms = 32;     // atoms per move instance

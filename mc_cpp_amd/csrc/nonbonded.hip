@@ -277,11 +277,11 @@ __global__ __launch_bounds__(64 * kSwWaves) void k_switch_energy(DevSys s, doubl
     }
 }
 
-__global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ partial, int nblocks, int ncomp,
-                                                 double *__restrict__ out)
+// thread t sums blocks t, t+256, ... of component c in order, then a
+// fixed-shape tree: the result does not depend on scheduling.
+__device__ __forceinline__ void reduce_partials(const double *__restrict__ partial, int nblocks, int ncomp,
+                                                double *__restrict__ out)
 {
-    // thread t sums blocks t, t+256, ... of component c in order, then a
-    // fixed-shape tree: the result does not depend on scheduling.
     __shared__ double sm[256];
     for (int c = 0; c < ncomp; c++) {
         double v = 0.0;
@@ -295,6 +295,20 @@ __global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ parti
         if (threadIdx.x == 0) out[c] = sm[0];
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ partial, int nblocks, int ncomp,
+                                                 double *__restrict__ out)
+{
+    reduce_partials(partial, nblocks, ncomp, out);
+}
+
+// two independent sums in one launch, one block each
+__global__ __launch_bounds__(256) void k_reduce2(const double *__restrict__ pa, int na, int ca, double *__restrict__ oa,
+                                                  const double *__restrict__ pb, int nb, int cb, double *__restrict__ ob)
+{
+    if (blockIdx.x == 0) reduce_partials(pa, na, ca, oa);
+    else reduce_partials(pb, nb, cb, ob);
 }
 
 }  // namespace
@@ -367,4 +381,10 @@ void launch_switch_energy(const DevSys &s, double *partial, int nblocks, hipStre
 void launch_reduce(const double *partial, int nblocks, int ncomp, double *out, hipStream_t st)
 {
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, st, partial, nblocks, ncomp, out);
+}
+
+void launch_reduce2(const double *pa, int na, int ca, double *oa, const double *pb, int nb, int cb, double *ob,
+                    hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reduce2, dim3(2), dim3(256), 0, st, pa, na, ca, oa, pb, nb, cb, ob);
 }
