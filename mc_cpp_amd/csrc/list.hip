@@ -165,9 +165,13 @@ __global__ __launch_bounds__(kBinT) void k_bin_count(DevSys s, CellGrid g, int g
     __syncthreads();
     for (int c = threadIdx.x; c < nc; c += kBinT)
         if (hist[c]) atomicAdd(&g.cell_acc[c], hist[c]);
-    __threadfence();
+    // every wavefront's additions have completed at the barrier; one
+    // device-scope fence per block orders them before its ticket
     __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(g.ticket, 1u) == gridDim.x - 1;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(g.ticket, 1u) == gridDim.x - 1;
+    }
     __syncthreads();
     if (!last) return;
     __threadfence();
