@@ -701,3 +701,37 @@ def test_energy_over_the_list_of_the_last_update():
     eo2 = o.getE()
     _check_energy(eg2, eo2, "rebuilt list")
     assert eo[0] != eo2[0]
+
+
+def test_moved_coordinate_uploads_back_to_back():
+    """mcx_set_coords_moved stages x, y, z in one pinned buffer and makes the
+    device copy the master: two uploads with no energy call between them
+    leave the second (the staging buffer is not overwritten while a copy
+    from it is in flight); the coordinates read back are the uploaded ones
+    bit for bit; re-setting the atom parameters afterwards (which rebuilds
+    the fixed-point mirror from the coordinates) keeps them; every energy
+    equals the FP64 energy of those coordinates over the list of the last
+    update."""
+    s, _ = systems.water_box(seed=3, nside=8, L=24.85)
+    o, ref = _pair(s)
+    o.update_list(16)
+    ff = mcx.FField(s)
+    rng = np.random.default_rng(11)
+    moved = []
+    for k in range(2):
+        c = [a + rng.uniform(-0.3, 0.3, a.shape) for a in (s.x, s.y, s.z)]
+        moved.append(c)
+        ff.h.set_coords_moved(*c)
+    for _ in range(2):
+        ref.x[:], ref.y[:], ref.z[:] = moved[1]
+        _check_energy(ff.h.energy(), o.getE(), "second of two uploads")
+    x, y, z = ff.h.get_coords()
+    for a, b in zip((x, y, z), moved[1]):
+        assert np.array_equal(a, b)
+    ff.h.set_coords_moved(*moved[0])
+    ff.h.set_atoms(s.charge, s.eps, s.sig, s.eps14, s.sig14, s.mass, s.frozen, s.resid)
+    ref.x[:], ref.y[:], ref.z[:] = moved[0]
+    _check_energy(ff.h.energy(), o.getE(), "after the parameters were set again")
+    x, y, z = ff.h.get_coords()
+    for a, b in zip((x, y, z), moved[0]):
+        assert np.array_equal(a, b)
