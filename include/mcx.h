@@ -111,8 +111,11 @@ int mcx_set_coords(mcx_ctx *h, const double *x, const double *y, const double *z
  * AtomList in place (include/Move.hpp:42-143): the pair list stays the one
  * built at the last update, as the reference's Verlet list does until
  * askListUpdate (src/FField.cpp:222-235), so a following mcx_energy is the
- * reference's getE over that list.  The fixed-point mirror is recomputed on
- * the device.  The first upload of a handle behaves as mcx_set_coords. */
+ * reference's getE over that list.  The arrays go up in one copy from a
+ * pinned staging buffer of the handle (the call returns once they are
+ * staged; the next call waits for that copy) and the fixed-point mirror is
+ * recomputed on the device.  The first upload of a handle behaves as
+ * mcx_set_coords. */
 int mcx_set_coords_moved(mcx_ctx *h, const double *x, const double *y, const double *z);
 int mcx_get_coords(mcx_ctx *h, double *x, double *y, double *z);
 
