@@ -595,7 +595,7 @@ def test_gradient_vs_fp64_and_fd():
 
 
 @pytest.mark.parametrize("case", ["water1536", "bonded_mix", "water_small_box", "mixed_small_frozen", "water12288",
-                                  "water12288_every2500"])
+                                  "water12288_every2500", "water12288_long_rows"])
 def test_incremental_list_equals_full_rebuild(case, monkeypatch):
     """Scheduled rebuilds inside mcx_mc_run are incremental updates (rows of
     the atoms moved since the last update rebuilt, the other rows patched
@@ -607,7 +607,10 @@ def test_incremental_list_equals_full_rebuild(case, monkeypatch):
     move since the binning, which asks for a full build in between;
     water12288_every2500 updates every 2,500 moves, so that one update takes
     more than 1,024 moved atoms (more than one per thread of the row kernel,
-    more than one per block of its grid) and as many stale ones."""
+    more than one per block of its grid) and as many stale ones;
+    water12288_long_rows has a 16 A cutoff (18 A list radius), so that rows
+    hold more than 2,048 atoms (more than the row kernels read ahead or
+    merge in one pass)."""
     import copy
     if case == "water1536":
         s = systems.water_box(seed=3, nside=8, L=24.85)[0]
@@ -618,6 +621,9 @@ def test_incremental_list_equals_full_rebuild(case, monkeypatch):
     elif case == "mixed_small_frozen":
         s = systems.mixed_system(seed=3, nside=8, L=24.85, nbead=60)[0]
         s.frozen[::7] = 1
+    elif case == "water12288_long_rows":
+        s = systems.water_box(seed=2)[0]
+        s.ctoff, s.cton = 16.0, 14.0
     else:
         s = systems.water_box(seed=2)[0]
     n = {"water12288": 12000, "water12288_every2500": 5000}.get(case, 3000)
