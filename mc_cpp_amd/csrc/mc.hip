@@ -170,10 +170,83 @@ __device__ __forceinline__ bool set_holds(const MoveLDS &m, const int4 id)
 // the one-workgroup kernels pass threadIdx.x - 64, the wide kernel the
 // thread of the wavefront it stands in for when a proposal's pairs are
 // split over several workgroups.
+// A trial that leaves the list's validity (an atom beyond half the skin from
+// where the list was built), with a cutoff: every atom is a candidate.  The
+// atoms stream past each pair thread kScanU at a time; an atom farther from
+// moved atom 0's old position than the cutoff plus the set's old extent, and
+// farther from its new position than the cutoff plus the set's new extent
+// (FP32 minimum-image distances, a 0.01 A margin), has no pair within the
+// cutoff of any moved atom before or after and is skipped; the others go
+// through the exact pair change as the list entries do.
+constexpr int kScanU = 4;
+
+__device__ __forceinline__ void delta_pairs_scan(const DevSys &s, const MoveLDS &m, bool fb, double &dec,
+                                                 double &dev, double (&fnew)[3], int pt)
+{
+    constexpr int NP = kMCT - 64;
+    const int nm = m.nm;
+    const int4 c0 = m.ip[0];
+    const float ux = (float)s.unit[0], uy = (float)s.unit[1], uz = (float)s.unit[2];
+    const float Lx = (float)s.L[0], Ly = (float)s.L[1], Lz = (float)s.L[2];
+    const float rLx = (float)s.rL[0], rLy = (float)s.rL[1], rLz = (float)s.rL[2];
+    const float d0x = (float)m.dx[0], d0y = (float)m.dy[0], d0z = (float)m.dz[0];
+    // the set's extent about atom 0, old and new
+    float eo2 = 0.f, en2 = 0.f;
+    for (int a = 1; a < nm; a++) {
+        const float ox = (float)mcx_wdiff(m.ip[a].x, c0.x) * ux, oy = (float)mcx_wdiff(m.ip[a].y, c0.y) * uy,
+                    oz = (float)mcx_wdiff(m.ip[a].z, c0.z) * uz;
+        eo2 = fmaxf(eo2, fmaf(oz, oz, fmaf(oy, oy, ox * ox)));
+        float nx = ox + ((float)m.dx[a] - d0x), ny = oy + ((float)m.dy[a] - d0y), nz = oz + ((float)m.dz[a] - d0z);
+        nx -= Lx * rintf(nx * rLx);
+        ny -= Ly * rintf(ny * rLy);
+        nz -= Lz * rintf(nz * rLz);
+        en2 = fmaxf(en2, fmaf(nz, nz, fmaf(ny, ny, nx * nx)));
+    }
+    const float rc = sqrtf((float)s.ctoff2);
+    const float ro = rc + sqrtf(eo2) + 0.01f, rn = rc + sqrtf(en2) + 0.01f;
+    const float ro2 = ro * ro, rn2 = rn * rn;
+    for (int j0 = 0; j0 < s.n; j0 += NP * kScanU) {
+        int4 pj[kScanU];
+#pragma unroll
+        for (int u = 0; u < kScanU; u++) {
+            const int j = j0 + u * NP + pt;
+            pj[u] = s.ipos[j < s.n ? j : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < kScanU; u++) {
+            const int j = j0 + u * NP + pt;
+            if (j >= s.n) continue;
+            const float ox = (float)mcx_wdiff(pj[u].x, c0.x) * ux, oy = (float)mcx_wdiff(pj[u].y, c0.y) * uy,
+                        oz = (float)mcx_wdiff(pj[u].z, c0.z) * uz;
+            float nx = ox - d0x, ny = oy - d0y, nz = oz - d0z;
+            nx -= Lx * rintf(nx * rLx);
+            ny -= Ly * rintf(ny * rLy);
+            nz -= Lz * rintf(nz * rLz);
+            if (fmaf(oz, oz, fmaf(oy, oy, ox * ox)) > ro2 && fmaf(nz, nz, fmaf(ny, ny, nx * nx)) > rn2) continue;
+            const double4 pq = s.qes[j];
+            for (int a = 0; a < nm; a++) {
+                const int i = m.first + a;
+                if (j == i || excluded(s, i, j)) continue;
+                if (s.frozen[i] && s.frozen[j]) continue;   // not a list pair
+                int b = -1;
+                if (j >= m.first && j <= m.last) {
+                    if (j < i) continue;     // pair of two moved atoms: counted once
+                    b = j - m.first;
+                }
+                pair_delta(s, m, i, a, j, b, pj[u], pq, dec, dev, fb, fnew);
+            }
+        }
+    }
+}
+
 __device__ __forceinline__ void delta_pairs(const DevSys &s, const MoveLDS &m, bool fb, double &dec, double &dev,
                                             double (&fnew)[3], int pt = threadIdx.x - 64)
 {
     constexpr int NP = kMCT - 64;
+    if (m.scan && s.cutmode != 0) {
+        delta_pairs_scan(s, m, fb, dec, dev, fnew, pt);
+        return;
+    }
     const int total = m.total, nm = m.nm;
     const bool all = s.cutmode == 0 || m.scan;
     // Row starts of a set of up to four atoms, read once: the moved atom of
