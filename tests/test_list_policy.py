@@ -51,9 +51,17 @@ def _check_list(h, o):
     assert np.array_equal(flat_o, flat_g)
 
 
-@pytest.mark.parametrize("name", ["water1536", "mixed_small", "water_small_box"])
+@pytest.mark.parametrize("name", ["water1536", "mixed_small", "water_small_box", "bonded_mix_large_steps"])
 def test_auto_list_matches_exact_cutoff_oracle(name):
-    if name == "water1536":
+    if name == "bonded_mix_large_steps":
+        # orthorhombic, every bonded kind, residue and atom moves large
+        # enough that many trials leave the list's validity (the filtered
+        # scan of every atom)
+        s, _ = systems.bonded_mix(seed=7)
+        for m, d in zip(s.moves, (1.2, 40.0, 0.9, 20.0)):
+            m.dmax = d
+        n = 2000
+    elif name == "water1536":
         # steps large enough that accepted moves outrun half the skin
         s, _ = systems.water_box(seed=3, nside=8, L=24.85)
         s.moves = [systems.MoveType(systems.MV_TRN, systems.MODE_RESIDUE, 1.2, 50.0, 0),
