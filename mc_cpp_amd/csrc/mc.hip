@@ -1006,9 +1006,12 @@ __device__ __forceinline__ bool round_barrier(const WideArgs &w, unsigned target
         __hip_atomic_fetch_add(w.sync, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const long long t0 = (long long)wall_clock64();
         int fail = 0;
-        while (__hip_atomic_load(w.sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if ((long long)wall_clock64() - t0 > w.spin_ticks ||
-                __hip_atomic_load(w.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        for (;;) {
+            // the count and the give-up flag read together: one round trip per poll
+            const unsigned v = __hip_atomic_load(w.sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int e = __hip_atomic_load(w.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (v >= target) break;
+            if (e != 0 || (long long)wall_clock64() - t0 > w.spin_ticks) {
                 fail = 1;
                 __hip_atomic_store(w.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
