@@ -374,7 +374,7 @@ int mcx_batch_state(mcx_batch *b, double *etot, int *slot_of_rep, long long *tri
  * (k, k+1), nrep-1 entries each, since the last reset. */
 int mcx_batch_exchange_stats(mcx_batch *b, long long *attempts, long long *accepted);
 
-/* Launch shape: threads per replica workgroup (256, 512 or 1024) and the
+/* Launch shape: threads per replica workgroup (256, 512, 768 or 1024) and the
  * number of 64-candidate tiles each wavefront keeps in flight (1, 2, 4). */
 int mcx_batch_set_config(mcx_batch *b, int threads, int unroll);
 

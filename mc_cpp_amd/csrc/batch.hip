@@ -638,6 +638,7 @@ BatchKernel variant(int threads, int unroll, bool img, bool cs = false)
                    : (cs ? kvar<NT, U, false, true>() : kvar<NT, U, false, false>());
     MCX_BV(256, 1) MCX_BV(256, 2) MCX_BV(256, 4)
     MCX_BV(512, 1) MCX_BV(512, 2) MCX_BV(512, 4)
+    MCX_BV(768, 1) MCX_BV(768, 2) MCX_BV(768, 4)
     MCX_BV(1024, 1) MCX_BV(1024, 2) MCX_BV(1024, 4)
 #undef MCX_BV
     return nullptr;

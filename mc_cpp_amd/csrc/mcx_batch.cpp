@@ -625,7 +625,7 @@ int mcx_batch_set_config(mcx_batch *b, int threads, int unroll)
 {
     return bguard(b, [&] {
         if (!batch_variant_ok(threads, unroll, true))
-            throw mcx_error(MCX_ERR_ARG, "threads must be 256, 512 or 1024 and unroll 1, 2 or 4");
+            throw mcx_error(MCX_ERR_ARG, "threads must be 256, 512, 768 or 1024 and unroll 1, 2 or 4");
         b->threads = threads;
         b->unroll = unroll;
         b->prepare();
@@ -646,7 +646,7 @@ int mcx_batch_tune(mcx_batch *b, int *threads, int *unroll, double *ms_per_sweep
         HIPCHK(hipEventCreate(&e1));
         double best = 1e300;
         int bt = b->threads, bu = b->unroll;
-        const int T[3] = {256, 512, 1024}, U[3] = {1, 2, 4};
+        const int T[4] = {256, 512, 768, 1024}, U[3] = {1, 2, 4};
         try {
             for (int nt : T)
                 for (int u : U) {

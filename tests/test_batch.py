@@ -174,7 +174,7 @@ def test_batch_launch_shapes_agree():
     handles = [mcx.from_system(s) for s in reps]
     b = mcx.Batch(handles, reps[0].moves)
     base = None
-    for threads in (256, 512, 1024):
+    for threads in (256, 512, 768, 1024):
         for unroll in (1, 2, 4):
             for h, c in zip(handles, coords):
                 h.set_coords(*c)
@@ -266,7 +266,7 @@ def test_tuner_keeps_state():
     before = b.state()
     xb = [h.get_coords() for h in handles]
     threads, unroll, ms = b.tune()
-    assert threads in (256, 512, 1024) and unroll in (1, 2, 4) and ms > 0
+    assert threads in (256, 512, 768, 1024) and unroll in (1, 2, 4) and ms > 0
     after = b.state()
     assert np.array_equal(before["etot"], after["etot"])
     for h, c in zip(handles, xb):
