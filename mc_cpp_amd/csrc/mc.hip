@@ -1212,14 +1212,20 @@ __global__ __launch_bounds__(kMCT) void k_mc_wide(DevSys s0, DevTerms t, DevMove
                 while (np < wmax && p + 64 - pos <= kWideRing) {
                     // proposals already parsed: a compact loop (the parse and
                     // generation below are rare, and keeping them out of this
-                    // loop keeps its code short)
-                    while (p < eq_hi) {
-                        const int e = g.endq[p & (kWideRing - 1)];
-                        if (e < 0) break;
-                        if (lane == np) g_start_l = p;
-                        p += e + 1;
-                        np++;
-                        if (!(np < wmax && p + 64 - pos <= kWideRing)) break;
+                    // loop keeps its code short), in 32-bit offsets from pos
+                    {
+                        const int qhi = (int)(eq_hi - pos), wm = (int)wmax;
+                        const int rb = (int)(pos & (kWideRing - 1));
+                        int q = (int)(p - pos);
+                        while (q < qhi) {
+                            const int e = g.endq[(rb + q) & (kWideRing - 1)];
+                            if (e < 0) break;
+                            if (lane == np) g_start_l = pos + q;
+                            q += e + 1;
+                            np++;
+                            if (!(np < wm && q <= kWideRing - 64)) break;
+                        }
+                        p = pos + q;
                     }
                     if (!(np < wmax && p + 64 - pos <= kWideRing)) break;
                     if (p >= eq_hi) {
