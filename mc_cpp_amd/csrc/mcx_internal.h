@@ -192,7 +192,7 @@ size_t list_build_lds_bytes(int n);
 // explicitly, as their cells are stale.
 constexpr int kPatchEdits = 32;      // edits one row can take in one update
 constexpr int kPatchRows = 4096;     // moved atoms one update can take
-constexpr int kPatchStale = 2048;    // atoms moved since the binning before a full build is asked for
+constexpr int kPatchStale = 3072;    // atoms moved since the binning before a full build is asked for
 struct ListPatch {
     int *moved_flag, *moved_list, *moved_cnt;   // moved since the last update (flag, list, count)
     int *stale_flag, *stale_list, *stale_cnt;   // moved since the last full build

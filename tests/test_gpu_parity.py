@@ -603,7 +603,7 @@ def test_incremental_list_equals_full_rebuild(case, monkeypatch):
     same chain bit for bit, and the list after the run equals the FP64
     oracle's list at the final coordinates (bit-exact: the same pairs in the
     same order).  water_small_box has a single cell; mixed_small_frozen has
-    frozen atoms; water12288 runs long enough for more than 2,048 atoms to
+    frozen atoms; water12288 runs long enough for more than 1,536 atoms to
     move since the binning, which asks for a full build in between;
     water12288_every2500 updates every 2,500 moves, so that one update takes
     more than 1,024 moved atoms (more than one per thread of the row kernel,
