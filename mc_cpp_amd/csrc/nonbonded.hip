@@ -18,8 +18,9 @@
 //
 // Pair arithmetic: the displacement is the wrapping int32 difference of the
 // fixed-point fractional coordinates (the exact minimum image, no rint),
-// converted to FP64; r^2 in FP64; 1/r from the FP32 v_rsq estimate refined
-// by one FP64 Newton step; the Coulomb sum in FP64.  The LJ part uses the
+// converted to FP64; r^2 in FP64; 1/r from a hardware estimate (FP32 v_rsq
+// in the list kernel, FP64 v_rsq in the all-pair tiles) refined by one FP64
+// Newton step; the Coulomb sum in FP64.  The LJ part uses the
 // FP32 estimate of 1/r and FP32 arithmetic, folded into FP64 every 8 pairs.
 // FP64 FMAs issue at the FP32 rate on this GPU, so the Coulomb refinement
 // costs a few issue slots and buys the 1e-6 relative total that water's
